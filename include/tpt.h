@@ -1,0 +1,280 @@
+/* tpt.h — C ABI of the MI355X wavefront path tracer (libtpt.so).
+ *
+ * The reference (BlurryLight/tiny-path-tracer) has no FFI: its render path is
+ * two C++ entry points, and each C-ABI function below replaces one of them or
+ * the driver loop around them:
+ *
+ *   hitable::hit(r, t_min, t_max, rec)          headers/hitable.h:32-33
+ *     -> tpt_trace_closest / tpt_trace_closest_record   (batched, device)
+ *   color(r, world, light_shape, depth, max_depth)  headers/utils.h:26-27,
+ *   with the per-row sample loop of main.cpp:109-156
+ *     -> tpt_render                                 (one call per frame/pass)
+ *   the object graph built by the scene API (headers/ of the reference)
+ *     -> tpt_flat_scene (host arrays, produced by the C++ flattener
+ *        tpt::flatten in host/flatten.h) -> tpt_scene_create (device upload)
+ *
+ * Conventions: plain pointers and sizes; every function returns an int status
+ * (TPT_OK = 0, negative on error) and tpt_last_error() returns the message of
+ * the calling thread's last failure.  No C++ exception crosses this boundary.
+ * Pointers named d_* are device (HBM) pointers; `stream` is a hipStream_t
+ * (NULL = the default stream).  A scene handle may be used from one stream at
+ * a time; distinct handles are independent.
+ */
+#ifndef TPT_H
+#define TPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPT_ABI_VERSION 1
+
+enum tpt_status {
+  TPT_OK = 0,
+  TPT_ERR_INVALID = -1,     /* bad argument */
+  TPT_ERR_HIP = -2,         /* a HIP runtime call failed */
+  TPT_ERR_UNSUPPORTED = -3, /* hitable/material/texture type not on the device path */
+  TPT_ERR_NOMEM = -4,
+  TPT_ERR_NODEVICE = -5     /* no gfx950 device */
+};
+
+/* ---------------------------------------------------------------------------
+ * Flattened scene (host arrays).  Layout is documented per record; all
+ * records are 16-byte aligned so one lane fetches one record with dwordx4
+ * loads.
+ *
+ * World = a sequence of segments evaluated like the reference's
+ * hitable_list::hit (src/hitable_list.cc:38-51): each segment is intersected
+ * with t_max = the closest hit so far.  A segment is a pre-order range
+ * [start, end) of slots holding one bvh_node tree (or a single leaf).  Within
+ * a segment the device walks the slots in pre-order with skip links, which
+ * visits exactly the nodes bvh_node::hit visits (src/hitable.cc:63-90: both
+ * children, caller's t_max, no shrinking) in the same left-to-right order.
+ * ------------------------------------------------------------------------- */
+
+/* slot.b >> 28 */
+#define TPT_SLOT_BOX 0u
+#define TPT_SLOT_SPHERE 1u
+#define TPT_SLOT_RECT 2u
+#define TPT_SLOT_LIST 3u
+#define TPT_PAYLOAD_MASK 0x0FFFFFFFu
+#define TPT_NO_RIGHT 0x0FFFFFFFu
+
+/* Rect axis (normal axis): 0 = yz_rect (x = k), 1 = xz_rect (y = k), 2 = xy_rect (z = k). */
+
+/* 32-byte pre-order slot.
+ *  BOX:    f[0..2] box min, f[3..5] box max; a = skip (first slot after the
+ *          subtree); b payload = index of the right child slot, or TPT_NO_RIGHT
+ *          for an n==1 node (left == right in the reference).
+ *  SPHERE: f[0..2] centre, f[3] radius; a = material; b payload = leaf id.
+ *  RECT:   f[0] k, f[1] a0, f[2] a1, f[3] b0, f[4] b1 (in-plane bounds, in
+ *          the order of the reference constructor); a = material | axis<<28 |
+ *          flip<<30; b payload = leaf id.
+ *  LIST:   a = first entry; f[0] = leaf id of the first entry (as uint32 bits),
+ *          f[1] = entry count (as uint32 bits).  A leaf made of wrappers
+ *          (translate / rotate_y / flip_normal), box and hitable_list around
+ *          primitives; evaluated with the list's shrinking t_max.
+ * Leaf ids number primitive occurrences in depth-first order of the object
+ * graph (bvh left before right, lists in order, an n==1 node's duplicate child
+ * counted once); tpt_trace_closest reports them. */
+typedef struct tpt_slot {
+  float f[6];
+  uint32_t a;
+  uint32_t b;
+} tpt_slot;
+
+/* 32-byte entry of a LIST leaf: one primitive seen through a wrapper chain.
+ *  f[0..4] primitive parameters as in tpt_slot (sphere: centre, radius;
+ *  rect: k, a0, a1, b0, b1); xform = index into xforms or 0xFFFFFFFF;
+ *  a = material | axis<<28 | flip<<30 (flip = parity of flip_normal wrappers);
+ *  b = TPT_SLOT_SPHERE or TPT_SLOT_RECT. */
+typedef struct tpt_entry {
+  float f[5];
+  uint32_t xform;
+  uint32_t a;
+  uint32_t b;
+} tpt_entry;
+
+/* Wrapper chain, outermost first, at most 4 operations.  kinds: 2 bits per op
+ * (0 = translate by p[i][0..2], 1 = rotate_y with p[i][0] = sin, p[i][1] = cos,
+ * values from rotate_y's constructor). */
+#define TPT_XFORM_TRANSLATE 0u
+#define TPT_XFORM_ROTATE_Y 1u
+#define TPT_XFORM_MAX_OPS 4
+typedef struct tpt_xform {
+  uint32_t nops;
+  uint32_t kinds;
+  float p[TPT_XFORM_MAX_OPS][3];
+  uint32_t pad[2];
+} tpt_xform;
+
+#define TPT_MAT_LAMBERTIAN 0u
+#define TPT_MAT_METAL 1u
+#define TPT_MAT_DIELECTRIC 2u
+#define TPT_MAT_DIFFUSE_LIGHT 3u
+/* lambertian: tex = albedo texture.  metal: p[0..2] albedo, p[3] fuzz.
+ * dielectric: p[0] ior.  diffuse_light: tex = emission texture. */
+typedef struct tpt_material {
+  uint32_t kind;
+  uint32_t tex;
+  float p[6];
+} tpt_material;
+
+#define TPT_TEX_CONSTANT 0u
+#define TPT_TEX_CHECKER 1u
+#define TPT_TEX_PERLIN 2u
+#define TPT_TEX_IMAGE 3u
+/* constant: p[0..2] colour.  checker: a = odd, b = even.  perlin: p[0] scale,
+ * a = octaves.  image: a = byte offset into texels, b = width, c = height. */
+typedef struct tpt_texture {
+  uint32_t kind;
+  uint32_t a, b, c;
+  float p[4];
+} tpt_texture;
+
+/* Perlin tables (shared by every Perlin texture of a scene, as the static
+ * tables of headers/perlin_noise.h:13-16 are). */
+typedef struct tpt_perlin {
+  float grad[256][4];
+  int32_t perm[3][256];
+} tpt_perlin;
+
+#define TPT_LIGHT_NONE 0u    /* hitable without pdf_value/random overrides: pdf 0, direction (1,0,0) */
+#define TPT_LIGHT_XZ_RECT 1u /* p[0..4] = x0, x1, z0, z1, k */
+#define TPT_LIGHT_SPHERE 2u  /* p[0..3] = centre, radius */
+typedef struct tpt_light {
+  uint32_t kind;
+  float weight; /* product of the 1/list_size factors above it */
+  float p[6];
+} tpt_light;
+
+typedef struct tpt_flat_scene {
+  const tpt_slot *slots;
+  uint32_t n_slots;
+  const uint32_t *segments; /* n_segments pairs (start, end) */
+  uint32_t n_segments;
+  const tpt_entry *entries;
+  uint32_t n_entries;
+  const tpt_xform *xforms;
+  uint32_t n_xforms;
+  const tpt_material *materials;
+  uint32_t n_materials;
+  const tpt_texture *textures;
+  uint32_t n_textures;
+  const uint8_t *texels;
+  uint64_t n_texels;
+  const tpt_perlin *perlin; /* NULL when no Perlin texture */
+  const tpt_light *lights;  /* light list flattened; n_lights == 0: no light sampling */
+  uint32_t n_lights;
+  uint32_t n_leaves; /* number of leaf ids */
+  uint32_t max_depth; /* deepest bvh_node nesting */
+} tpt_flat_scene;
+
+/* Camera fields as computed by camera_with_blur's constructor
+ * (src/camera.cc:2-21); ray generation follows get_ray (src/camera.cc:23-31). */
+typedef struct tpt_camera {
+  float origin[3];
+  float lower_left[3];
+  float horizontal[3];
+  float vertical[3];
+  float u[3], v[3], w[3];
+  float lens_radius;
+  float time0, time1;
+} tpt_camera;
+
+/* ---------------------------------------------------------- scene library ---
+ * Counterpart of the reference's scene factories (random_scene(),
+ * cornell_box(), ... headers/utils.h:29-36) plus the benchmark scenes.  Builds
+ * the named scene through the C++ scene API on a fresh thread whose drand_r
+ * generator is seeded with `seed` (5489 = the reference's default state),
+ * flattens it and keeps the host arrays.  Names (tpt_scene_names()):
+ *   random_spheres          one-weekend spheres, 400x400, 16 spp, depth 50
+ *   random_spheres_textured the same geometry with checker/Perlin/image
+ *                           textures, 1600x1600, 300 spp, depth 50
+ *   cornell555              the 555-unit Cornell box, 1200x1200, 2048 spp
+ *   random_spheres_n<N>     random spheres with about N small spheres
+ *   ref_cornell_box, ref_sphere_cornell_box, ref_two_perlin_spheres,
+ *   ref_light_spheres, ref_two_checker_spheres   the reference's scenes
+ *   box_list                instanced boxes in a plain hitable_list */
+typedef struct tpt_host_scene tpt_host_scene;
+typedef struct tpt_scene_info {
+  uint32_t width, height, spp, max_depth; /* the scene's default render settings */
+  float background;                       /* sky scale used when the scene has no light */
+  uint32_t n_leaves;
+} tpt_scene_info;
+
+const char *tpt_scene_names(void);
+int tpt_host_scene_build(const char *name, uint32_t seed, tpt_host_scene **out);
+int tpt_host_scene_flat(const tpt_host_scene *hs, tpt_flat_scene *flat);
+int tpt_host_scene_camera(const tpt_host_scene *hs, tpt_camera *camera);
+int tpt_host_scene_info(const tpt_host_scene *hs, tpt_scene_info *info);
+int tpt_host_scene_destroy(tpt_host_scene *hs);
+
+/* ------------------------------------------------------------- lifecycle --- */
+typedef struct tpt_scene tpt_scene;
+
+int tpt_abi_version(void);
+const char *tpt_last_error(void);
+/* Number of visible gfx950 devices (0 when none); never fails. */
+int tpt_device_count(void);
+
+/* Upload a flattened scene to the current device.  The flat arrays are copied;
+ * the caller may free them afterwards. */
+int tpt_scene_create(const tpt_flat_scene *flat, tpt_scene **out);
+int tpt_scene_destroy(tpt_scene *scene);
+
+/* ------------------------------------------------------------ closest hit ---
+ * Replaces hitable::hit (headers/hitable.h:32-33) for a batch of n rays in SoA
+ * layout: d_rays = 6 arrays of n floats back to back (ox, oy, oz, dx, dy, dz).
+ * Outputs per ray: d_leaf = leaf id of the closest hit or -1 on a miss, d_t =
+ * hit distance (undefined on a miss).  Results equal the reference's
+ * recursive hit() bit for bit (same IEEE single operations, FP64 sphere roots,
+ * the same tie and NaN rules). */
+int tpt_trace_closest(tpt_scene *scene, const float *d_rays, uint64_t n, float t_min, float t_max,
+                      int32_t *d_leaf, float *d_t, void *stream);
+
+/* As tpt_trace_closest, and also writes the whole hit_record: d_rec = 8 arrays
+ * of n floats (u, v, px, py, pz, nx, ny, nz) and d_mat = material index. */
+int tpt_trace_closest_record(tpt_scene *scene, const float *d_rays, uint64_t n, float t_min,
+                             float t_max, int32_t *d_leaf, float *d_t, float *d_rec,
+                             int32_t *d_mat, void *stream);
+
+/* ----------------------------------------------------------------- render ---
+ * Replaces color() (src/utils.cc:58-94) driven by the sample loop of
+ * main.cpp:115-134: renders spp samples per pixel, sample indices
+ * [sample_offset, sample_offset + spp), and ADDS per-pixel sums to the
+ * caller's device buffers: d_sum (3·width·height floats, RGB interleaved,
+ * row 0 = bottom row j = 0 as in the reference) and d_sumsq (same shape,
+ * sum of squares; may be NULL).  Each sample is de_nan'ed per channel
+ * (headers/utils.h:100-109) before it is added. */
+typedef struct tpt_render_desc {
+  uint32_t width, height;
+  uint32_t spp;
+  uint32_t max_depth;     /* color()'s depth < max_depth gate */
+  uint64_t seed;          /* Philox key */
+  uint32_t sample_offset; /* first sample index of this pass */
+  float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
+  uint32_t guiding;       /* 0: reference estimator; 1: learned proposal mixed in by one-sample MIS */
+  uint32_t queue_capacity; /* paths in flight; 0 = automatic */
+  uint32_t flags;          /* reserved, must be 0 */
+} tpt_render_desc;
+
+typedef struct tpt_render_stats {
+  uint64_t paths;          /* camera paths completed */
+  uint64_t segments;       /* closest-hit queries (ray segments) */
+  uint64_t light_queries;  /* light-pdf intersections */
+  uint64_t nan_samples;    /* samples with a NaN channel zeroed by de_nan */
+  double kernel_ms;        /* summed kernel time of the pass */
+} tpt_render_stats;
+
+int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+               float *d_sumsq, tpt_render_stats *stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPT_H */

@@ -1,0 +1,335 @@
+// ORACLE — test infrastructure only (see oracle.h).
+// C entry points of liboracle.so for the tests: builds scenes with the
+// product's scene code, then intersects / renders them with the restatement.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../tiny-path-tracer_amd/host/scenes.h"
+#include "../tiny-path-tracer_amd/scene/hitable_list.h"
+#include "../tiny-path-tracer_amd/scene/material.h"
+#include "../tiny-path-tracer_amd/scene/rect_box.h"
+#include "../tiny-path-tracer_amd/scene/sphere.h"
+#include "oracle.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct philox_rng : oracle::rng {
+  oracle::path_stream s;
+  double next() override { return s.next(); }
+  void bounce(uint32_t b) override { s.set_bounce(b); }
+};
+
+struct mt_rng : oracle::rng {
+  std::mt19937 gen;  // default seed 5489, as a fresh row thread of the reference
+  std::uniform_real_distribution<double> dist{0.0, 1.0};
+  double next() override { return dist(gen); }
+};
+
+template <class F>
+void parallel_for(uint64_t n, int nthreads, F &&fn) {
+  if (nthreads < 1) nthreads = 1;
+  std::atomic<uint64_t> next{0};
+  const uint64_t chunk = 1024;
+  auto worker = [&]() {
+    for (;;) {
+      const uint64_t b = next.fetch_add(chunk);
+      if (b >= n) return;
+      const uint64_t e = std::min(n, b + chunk);
+      for (uint64_t i = b; i < e; ++i) fn(i);
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int t = 1; t < nthreads; ++t) ts.emplace_back(worker);
+  worker();
+  for (auto &t : ts) t.join();
+}
+
+// World-space primitives and boxes used to aim adversarial rays.
+struct aim_targets {
+  std::vector<AABB> boxes;
+  struct rect {
+    int axis;
+    float k, a0, a1, b0, b1;
+  };
+  std::vector<rect> rects;
+  std::vector<std::pair<vec3, float>> spheres;
+};
+
+void gather_targets(const hitable *h, vec3 offset, bool translated_only, aim_targets *t) {
+  if (auto *b = dynamic_cast<const bvh_node *>(h)) {
+    t->boxes.push_back(b->box_);
+    gather_targets(b->left_, offset, translated_only, t);
+    if (b->right_ != b->left_) gather_targets(b->right_, offset, translated_only, t);
+  } else if (auto *l = dynamic_cast<const hitable_list *>(h)) {
+    for (int i = 0; i < l->list_size_; ++i) gather_targets(l->list_[i], offset, translated_only, t);
+  } else if (auto *bx = dynamic_cast<const box *>(h)) {
+    gather_targets(bx->list_ptr_, offset, translated_only, t);
+  } else if (auto *f = dynamic_cast<const flip_normal *>(h)) {
+    gather_targets(f->ptr_, offset, translated_only, t);
+  } else if (auto *tr = dynamic_cast<const translate *>(h)) {
+    gather_targets(tr->ptr_, offset + tr->offset_, translated_only, t);
+  } else if (dynamic_cast<const rotate_y *>(h)) {
+    // rotated content: aim only at its box
+    AABB bb;
+    if (h->bounding_box(0, 0, bb)) t->boxes.push_back(AABB(bb.min_ + offset, bb.max_ + offset));
+  } else if (auto *s = dynamic_cast<const sphere *>(h)) {
+    t->spheres.push_back({s->center_ + offset, s->radius_});
+  } else if (auto *q = dynamic_cast<const xy_rect *>(h)) {
+    t->rects.push_back({2, q->k_ + offset.z(), q->x0_ + offset.x(), q->x1_ + offset.x(), q->y0_ + offset.y(),
+                        q->y1_ + offset.y()});
+  } else if (auto *q2 = dynamic_cast<const xz_rect *>(h)) {
+    t->rects.push_back({1, q2->k_ + offset.y(), q2->x0_ + offset.x(), q2->x1_ + offset.x(),
+                        q2->z0_ + offset.z(), q2->z1_ + offset.z()});
+  } else if (auto *q3 = dynamic_cast<const yz_rect *>(h)) {
+    t->rects.push_back({0, q3->k_ + offset.x(), q3->y0_ + offset.y(), q3->y1_ + offset.y(),
+                        q3->z0_ + offset.z(), q3->z1_ + offset.z()});
+  }
+  (void)translated_only;
+}
+
+}  // namespace
+
+struct oracle_scene {
+  tpt::scene_spec spec;
+  oracle::graph_info g;
+  AABB world_box;
+};
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+int oracle_scene_build(const char *name, uint32_t seed, oracle_scene **out) {
+  auto *s = new oracle_scene();
+  std::string err;
+  if (!tpt::build_scene(name, seed, &s->spec, &err)) {
+    g_err = err;
+    delete s;
+    return -1;
+  }
+  oracle::index_graph(s->spec.world, &s->g);
+  if (!s->spec.world->bounding_box(0, 1, s->world_box)) s->world_box = AABB(vec3(-1, -1, -1), vec3(1, 1, 1));
+  *out = s;
+  return 0;
+}
+
+void oracle_scene_destroy(oracle_scene *s) { delete s; }
+
+uint32_t oracle_leaf_count(const oracle_scene *s) { return s->g.leaf_count.at(s->spec.world); }
+
+int oracle_scene_settings(const oracle_scene *s, uint32_t *w, uint32_t *h, uint32_t *spp, uint32_t *depth,
+                          float *bg) {
+  *w = s->spec.width;
+  *h = s->spec.height;
+  *spp = s->spec.spp;
+  *depth = s->spec.max_depth;
+  *bg = s->spec.background;
+  return 0;
+}
+
+// 24 floats: origin, lower_left, horizontal, vertical, u, v, w, then
+// lens_radius, time0, time1.
+int oracle_camera_fields(const oracle_scene *s, float *out) {
+  const camera_with_blur &c = *s->spec.cam;
+  const vec3 *vs[7] = {&c.origin_, &c.lower_left_corner_, &c.horizontal_, &c.vertical_, &c.u_, &c.v_, &c.w_};
+  for (int i = 0; i < 7; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = vs[i]->e[j];
+  out[21] = c.lens_radius_;
+  out[22] = c.time0;
+  out[23] = c.time1;
+  return 0;
+}
+
+// rays: 6 arrays of n floats (ox, oy, oz, dx, dy, dz).  rec: 8 arrays of n
+// (u, v, px, py, pz, nx, ny, nz); rec and mat may be null.
+int oracle_trace(const oracle_scene *s, const float *rays, uint64_t n, float tmin, float tmax, int32_t *leaf,
+                 float *t, float *rec, int32_t *mat, int nthreads) {
+  parallel_for(n, nthreads, [&](uint64_t i) {
+    const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]), vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+    hit_record hr;
+    uint32_t lf = 0;
+    if (oracle::hit(s->spec.world, s->g, r, tmin, tmax, hr, 0, &lf)) {
+      leaf[i] = static_cast<int32_t>(lf);
+      t[i] = hr.t;
+      if (rec) {
+        const float vals[8] = {hr.u, hr.v, hr.point.x(), hr.point.y(), hr.point.z(),
+                               hr.normal.x(), hr.normal.y(), hr.normal.z()};
+        for (int k = 0; k < 8; ++k) rec[k * n + i] = vals[k];
+      }
+      if (mat) mat[i] = static_cast<int32_t>(s->g.material_id.at(hr.mat_ptr));
+    } else {
+      leaf[i] = -1;
+      t[i] = 0.0f;
+      if (rec)
+        for (int k = 0; k < 8; ++k) rec[k * n + i] = 0.0f;
+      if (mat) mat[i] = -1;
+    }
+  });
+  return 0;
+}
+
+// kind 0: camera primaries; 1: one-bounce secondaries scattered by the
+// oracle; 2: origins uniform in the (10% grown) world box, directions uniform
+// on the sphere times a random length in [0.25, 4); 3: adversarial rays
+// (axis-aligned, in-plane, at box corners/edges, sphere tangents, extreme
+// magnitudes).  Deterministic in (kind, seed, i).
+int oracle_make_rays(const oracle_scene *s, int kind, uint64_t n, uint64_t seed, float *rays, int nthreads) {
+  const camera_with_blur &cam = *s->spec.cam;
+  const uint32_t W = s->spec.width, H = s->spec.height;
+  aim_targets tg;
+  if (kind == 3) gather_targets(s->spec.world, vec3(0, 0, 0), true, &tg);
+  const vec3 lo = s->world_box.min_, hi = s->world_box.max_;
+  const vec3 ext = hi - lo;
+  parallel_for(n, nthreads, [&](uint64_t i) {
+    philox_rng R;
+    R.s.pixel = static_cast<uint32_t>(i);
+    R.s.sample = static_cast<uint32_t>(i >> 32) ^ 0xA5A5u;
+    R.s.k0 = static_cast<uint32_t>(seed);
+    R.s.k1 = static_cast<uint32_t>(seed >> 32) ^ static_cast<uint32_t>(kind * 0x1234567);
+    R.s.set_bounce(0);
+    ray out;
+    auto primary = [&]() {
+      const uint32_t px = static_cast<uint32_t>(R.next() * W);
+      const uint32_t py = static_cast<uint32_t>(R.next() * H);
+      const float u = static_cast<float>((static_cast<float>(px) + R.next()) / static_cast<float>(W));
+      const float v = static_cast<float>((static_cast<float>(py) + R.next()) / static_cast<float>(H));
+      return oracle::camera_ray(cam, u, v, R);
+    };
+    if (kind == 0) {
+      out = primary();
+    } else if (kind == 1) {
+      out = primary();
+      for (int tries = 0; tries < 8; ++tries) {
+        hit_record rec;
+        uint32_t lf;
+        if (oracle::hit(s->spec.world, s->g, out, 0.001f, std::numeric_limits<float>::max(), rec, 0, &lf)) {
+          ray next;
+          if (oracle::scatter_once(out, rec, s->g, s->spec.lights, R, &next)) {
+            out = next;
+            break;
+          }
+        }
+        out = primary();
+      }
+    } else if (kind == 2 || (kind == 3 && tg.boxes.empty() && tg.rects.empty() && tg.spheres.empty())) {
+      const vec3 o(lo.x() - 0.05f * ext.x() + static_cast<float>(R.next()) * 1.1f * ext.x(),
+                   lo.y() - 0.05f * ext.y() + static_cast<float>(R.next()) * 1.1f * ext.y(),
+                   lo.z() - 0.05f * ext.z() + static_cast<float>(R.next()) * 1.1f * ext.z());
+      const float z = static_cast<float>(1 - 2 * R.next());
+      const float phi = static_cast<float>(2 * M_PI * R.next());
+      const float rr = std::sqrt(std::max(0.0f, 1 - z * z));
+      const float len = static_cast<float>(std::exp2(-2.0 + 4.0 * R.next()));
+      out = ray(o, len * vec3(rr * std::cos(phi), rr * std::sin(phi), z));
+    } else {
+      const int sub = static_cast<int>(R.next() * 8);
+      auto rnd_in = [&](float a, float b) { return a + static_cast<float>(R.next()) * (b - a); };
+      auto rnd_origin = [&]() {
+        return vec3(rnd_in(lo.x() - 0.2f * ext.x(), hi.x() + 0.2f * ext.x()),
+                    rnd_in(lo.y() - 0.2f * ext.y(), hi.y() + 0.2f * ext.y()),
+                    rnd_in(lo.z() - 0.2f * ext.z(), hi.z() + 0.2f * ext.z()));
+      };
+      vec3 o = rnd_origin();
+      vec3 d;
+      if (sub == 0) {  // axis-aligned, signed zeros in the other components
+        const int ax = static_cast<int>(R.next() * 3);
+        d = vec3(R.next() < 0.5 ? 0.0f : -0.0f, R.next() < 0.5 ? 0.0f : -0.0f, R.next() < 0.5 ? 0.0f : -0.0f);
+        d[ax] = R.next() < 0.5 ? 1.0f : -1.0f;
+      } else if (sub == 1 && !tg.rects.empty()) {  // in the plane of a rect (NaN t)
+        const auto &q = tg.rects[static_cast<size_t>(R.next() * tg.rects.size())];
+        o[q.axis] = q.k;
+        d = vec3(static_cast<float>(R.next() - 0.5), static_cast<float>(R.next() - 0.5),
+                 static_cast<float>(R.next() - 0.5));
+        d[q.axis] = R.next() < 0.5 ? 0.0f : -0.0f;
+      } else if (sub == 2 && !tg.boxes.empty()) {  // at a box corner
+        const AABB &b = tg.boxes[static_cast<size_t>(R.next() * tg.boxes.size())];
+        const vec3 c(R.next() < 0.5 ? b.min_.x() : b.max_.x(), R.next() < 0.5 ? b.min_.y() : b.max_.y(),
+                     R.next() < 0.5 ? b.min_.z() : b.max_.z());
+        d = c - o;
+      } else if (sub == 3 && !tg.rects.empty()) {  // at a rect edge or corner
+        const auto &q = tg.rects[static_cast<size_t>(R.next() * tg.rects.size())];
+        vec3 p;
+        const int i0 = q.axis == 0 ? 1 : 0, i1 = q.axis == 2 ? 1 : 2;
+        p[q.axis] = q.k;
+        p[i0] = R.next() < 0.5 ? q.a0 : q.a1;
+        p[i1] = R.next() < 0.3 ? (R.next() < 0.5 ? q.b0 : q.b1) : rnd_in(q.b0, q.b1);
+        d = p - o;
+      } else if (sub == 4 && !tg.spheres.empty()) {  // tangent to a sphere
+        const auto &sp = tg.spheres[static_cast<size_t>(R.next() * tg.spheres.size())];
+        const vec3 to = sp.first - o;
+        const vec3 perp = unit_vector(cross(to, vec3(0.3f, 1.0f, 0.7f)));
+        const vec3 touch = sp.first + sp.second * perp;
+        d = touch - o;
+      } else if (sub == 5 && !tg.rects.empty()) {  // origin exactly on a rect plane
+        const auto &q = tg.rects[static_cast<size_t>(R.next() * tg.rects.size())];
+        o[q.axis] = q.k;
+        d = vec3(static_cast<float>(R.next() - 0.5), static_cast<float>(R.next() - 0.5),
+                 static_cast<float>(R.next() - 0.5));
+      } else if (sub == 6) {  // extreme magnitudes
+        const float scale = R.next() < 0.5 ? 1e-30f : 1e30f;
+        d = scale * vec3(static_cast<float>(R.next() - 0.5), static_cast<float>(R.next() - 0.5),
+                         static_cast<float>(R.next() - 0.5));
+      } else {  // far origin aimed at the scene
+        o = 1e5f * vec3(static_cast<float>(R.next() - 0.5), static_cast<float>(R.next() - 0.5),
+                        static_cast<float>(R.next() - 0.5));
+        d = rnd_origin() - o;
+      }
+      out = ray(o, d);
+    }
+    for (int k = 0; k < 3; ++k) {
+      rays[k * n + i] = out.origin()[k];
+      rays[(3 + k) * n + i] = out.direction()[k];
+    }
+  });
+  return 0;
+}
+
+// Renders spp samples per pixel into sum/sumsq (3·w·h floats each, row 0 =
+// bottom, accumulated with +=).  rng_mode 0: the Philox path streams the
+// device uses; 1: a fresh default-seeded mt19937 per row (the reference's
+// thread-per-row generator state, main.cpp:157-159 + src/utils.cc:28-32).
+int oracle_render(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint64_t seed,
+                  uint32_t sample_offset, float background, int rng_mode, int nthreads, float *sum,
+                  float *sumsq) {
+  oracle::render_opts o;
+  o.max_depth = static_cast<int>(max_depth);
+  o.background = background;
+  parallel_for(h, nthreads, [&](uint64_t j) {
+    mt_rng mt;
+    philox_rng ph;
+    ph.s.k0 = static_cast<uint32_t>(seed);
+    ph.s.k1 = static_cast<uint32_t>(seed >> 32);
+    for (uint32_t i = 0; i < w; ++i) {
+      const uint64_t px = j * w + i;
+      for (uint32_t k = 0; k < spp; ++k) {
+        oracle::rng *R = &mt;
+        if (rng_mode == 0) {
+          ph.s.pixel = static_cast<uint32_t>(px);
+          ph.s.sample = sample_offset + k;
+          ph.s.set_bounce(0);
+          R = &ph;
+        }
+        const float u = static_cast<float>((static_cast<float>(i) + R->next()) / static_cast<float>(w));
+        const float v = static_cast<float>((static_cast<float>(j) + R->next()) / static_cast<float>(h));
+        const ray r = oracle::camera_ray(*s->spec.cam, u, v, *R);
+        const vec3 c = oracle::color(r, s->spec.world, s->g, s->spec.lights, 0, o, *R);
+        for (int ch = 0; ch < 3; ++ch) {
+          const float x = std::isnan(c[ch]) ? 0.0f : c[ch];
+          sum[px * 3 + ch] += x;
+          if (sumsq) sumsq[px * 3 + ch] += x * x;
+        }
+      }
+    }
+  });
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,338 @@
+// ORACLE — test infrastructure only (see oracle.h).
+// Restatement of color() and everything it reaches below hit().
+#include <cmath>
+#include <limits>
+
+#include "../tiny-path-tracer_amd/scene/hitable_list.h"
+#include "../tiny-path-tracer_amd/scene/material.h"
+#include "../tiny-path-tracer_amd/scene/rect_box.h"
+#include "../tiny-path-tracer_amd/scene/sphere.h"
+#include "../tiny-path-tracer_amd/scene/texture.h"
+#include "oracle.h"
+
+namespace oracle {
+
+namespace {
+
+const float kFltMax = std::numeric_limits<float>::max();
+
+// ------------------------------------------------------------- samplers ---
+// src/utils.cc:13-19.  Draw order within one vector: x then y.
+vec3 in_unit_disk(rng &R) {
+  vec3 p;
+  do {
+    const double a = R.next();
+    const double b = R.next();
+    p = 2.0f * vec3(static_cast<float>(a), static_cast<float>(b), 0) - vec3(1, 1, 0);
+  } while (dot(p, p) >= 1.0);
+  return p;
+}
+
+// src/utils.cc:21-27.  Draw order x, y, z.
+vec3 in_unit_sphere(rng &R) {
+  vec3 p;
+  do {
+    const double a = R.next();
+    const double b = R.next();
+    const double c = R.next();
+    p = 2.0f * vec3(static_cast<float>(a), static_cast<float>(b), static_cast<float>(c)) - vec3(1, 1, 1);
+  } while (p.length() >= 1.0);
+  return p;
+}
+
+// src/utils.cc:427-435.
+vec3 on_hemisphere(rng &R) {
+  const float r1 = static_cast<float>(R.next());
+  const float r2 = static_cast<float>(R.next());
+  const float phi = static_cast<float>(2 * M_PI * r1);
+  return vec3(std::cos(phi) * std::sqrt(r2), std::sin(phi) * std::sqrt(r2), std::sqrt(1 - r2));
+}
+
+// onb::build_from_w (src/utils.cc:437-450) and local() (headers/utils.h:55-58).
+struct onb {
+  vec3 u, v, w;
+  explicit onb(const vec3 &n) {
+    w = n;
+    const vec3 helper = std::abs(n.x()) > 0.9 ? vec3(0, 1, 0) : vec3(1, 0, 0);
+    v = unit_vector(cross(n, helper));
+    u = cross(v, w);
+  }
+  vec3 local(float a, float b, float c) const { return a * u + b * v + c * w; }
+};
+
+vec3 reflect(const vec3 &v, const vec3 &n) { return v - 2 * dot(v, n) * n; }
+
+// src/utils.cc:34-44; the unqualified sqrt there is ::sqrt(double).
+bool refract(const vec3 &v, const vec3 &n, float ni_over_nt, vec3 &refracted) {
+  const vec3 uv = unit_vector(v);
+  const float dt = dot(uv, n);
+  const float disc = static_cast<float>(1.0 - ni_over_nt * ni_over_nt * (1 - dt * dt));
+  if (disc > 0) {
+    refracted = ni_over_nt * (uv - n * dt) - n * static_cast<float>(::sqrt(static_cast<double>(disc)));
+    return true;
+  }
+  return false;
+}
+
+// src/utils.cc:46-51.
+float schlick(float cosine, float ior) {
+  float r0 = (1 - ior) / (1 + ior);
+  r0 = r0 * r0;
+  return static_cast<float>(r0 + (1 - r0) * std::pow(static_cast<double>(1 - cosine), 5));
+}
+
+// -------------------------------------------------------------- textures ---
+float fade(float x) {
+  const double d = x;
+  return static_cast<float>(6 * std::pow(d, 5) - 15 * std::pow(d, 4) + 10 * std::pow(d, 3));
+}
+
+// perlin_noise::noise + perlin_interpolate (src/utils.cc:160-191, 205-225).
+// The gradient offsets use the faded weights, as the reference does.
+float perlin(const vec3 &p) {
+  float u = p.x() - std::floor(p.x());
+  float v = p.y() - std::floor(p.y());
+  float w = p.z() - std::floor(p.z());
+  const int i = static_cast<int>(std::floor(p.x())) & 255;
+  const int j = static_cast<int>(std::floor(p.y())) & 255;
+  const int k = static_cast<int>(std::floor(p.z())) & 255;
+  u = fade(u);
+  v = fade(v);
+  w = fade(w);
+  float acc = 0.0f;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int c = 0; c < 2; c++) {
+        const vec3 &g = perlin_noise::random_vec3_[perlin_noise::permute_x_[(i + a) & 255] ^
+                                                    perlin_noise::permute_y_[(j + b) & 255] ^
+                                                    perlin_noise::permute_z_[(k + c) & 255]];
+        const vec3 off(u - a, v - b, w - c);
+        acc += (a * u + (1 - a) * (1 - u)) * (b * v + (1 - b) * (1 - v)) * (c * w + (1 - c) * (1 - w)) *
+               dot(g, off);
+      }
+  return std::abs(acc);
+}
+
+float turb(const vec3 &p, int depth) {
+  float acc = 0;
+  vec3 q = p;
+  float weight = 1.0f;
+  for (int i = 0; i < depth; i++) {
+    acc += weight * perlin(q);
+    weight *= 0.5f;
+    q *= 2;
+  }
+  return std::abs(acc);
+}
+
+vec3 tex_value(const texture *t, float u, float v, const vec3 &p) {
+  if (auto *c = dynamic_cast<const constant_texture *>(t)) return c->color_;
+  if (auto *ch = dynamic_cast<const checker_texture *>(t)) {
+    const float s = std::sin(10 * p.x()) * std::sin(10 * p.y()) * std::sin(10 * p.z());
+    return std::isless(s, 0.0f) ? tex_value(ch->odd_, u, v, p) : tex_value(ch->even_, u, v, p);
+  }
+  if (auto *pn = dynamic_cast<const perlin_noise_texture *>(t)) {
+    const float s = 1 + std::sin(pn->scale_ * p.z() + 10 * turb(p, pn->octaves_));
+    return vec3(1, 1, 1) * 0.5f * s;
+  }
+  if (auto *im = dynamic_cast<const image_texture *>(t)) {
+    int i = static_cast<int>(u * im->width_);
+    int j = static_cast<int>((1 - v) * im->height_);
+    if (i < 0) i = 0;
+    if (i > im->width_ - 1) i = im->width_ - 1;
+    if (j > im->height_ - 1) j = im->height_ - 1;
+    if (j < 0) j = 0;
+    const unsigned char *px = im->data_ + 3 * i + 3 * im->width_ * j;
+    return vec3(int(px[0]) / 255.0f, int(px[1]) / 255.0f, int(px[2]) / 255.0f);
+  }
+  return vec3(0, 0, 0);
+}
+
+// ------------------------------------------------------------ light pdfs ---
+// hitable::pdf_value / random of the light list (src/hitable_list.cc:24-36,
+// src/rect_box.cc:26-43, src/sphere.cc:93-120, headers/hitable.h:35-38).
+float light_pdf(const hitable *h, const graph_info &g, const vec3 &o, const vec3 &d) {
+  if (auto *l = dynamic_cast<const hitable_list *>(h)) {
+    const float weight = static_cast<float>(1.0 / l->list_size_);
+    float sum = 0;
+    for (int i = 0; i < l->list_size_; i++) sum += weight * light_pdf(l->list_[i], g, o, d);
+    return sum;
+  }
+  if (auto *r = dynamic_cast<const xz_rect *>(h)) {
+    hit_record rec;
+    uint32_t leaf;
+    if (hit(r, g, ray(o, d), 0.0001f, kFltMax, rec, 0, &leaf)) {
+      const float area = std::abs((r->x1_ - r->x0_) * (r->z1_ - r->z0_));
+      const float dist2 = (rec.t * d).squared_length();
+      const float cosine = std::abs(dot(d, rec.normal) / d.length());
+      return dist2 / (cosine * area);
+    }
+    return 0.0f;
+  }
+  if (auto *s = dynamic_cast<const sphere *>(h)) {
+    hit_record rec;
+    uint32_t leaf;
+    if (hit(s, g, ray(o, d), 0.001f, kFltMax, rec, 0, &leaf)) {
+      const float tmp = (s->radius_ * s->radius_) / (s->center_ - o).squared_length();
+      const float cmax = static_cast<float>(::sqrt(static_cast<double>(1 - tmp)));
+      const float solid = static_cast<float>(2 * M_PI * (1 - cmax));
+      if (std::isnan(solid)) return 0;
+      return 1 / solid;
+    }
+    return 0;
+  }
+  return 0.0f;
+}
+
+vec3 light_random(const hitable *h, const vec3 &o, rng &R) {
+  if (auto *l = dynamic_cast<const hitable_list *>(h)) {
+    const int idx = static_cast<int>(R.next() * l->list_size_);
+    return light_random(l->list_[idx], o, R);
+  }
+  if (auto *r = dynamic_cast<const xz_rect *>(h)) {
+    const double a = R.next();
+    const double b = R.next();
+    const vec3 pt(static_cast<float>(r->x0_ + a * (r->x1_ - r->x0_)), r->k_,
+                  static_cast<float>(r->z0_ + b * (r->z1_ - r->z0_)));
+    return pt - o;
+  }
+  if (auto *s = dynamic_cast<const sphere *>(h)) {
+    const vec3 dir = s->center_ - o;
+    const onb uvw(unit_vector(dir));
+    const float tmp = (s->radius_ * s->radius_) / (s->center_ - o).squared_length();
+    const float cmax = static_cast<float>(::sqrt(static_cast<double>(1 - tmp)));
+    const float r1 = static_cast<float>(R.next());
+    const float r2 = static_cast<float>(R.next());
+    const float z = 1 + r2 * (cmax - 1);
+    const float x = static_cast<float>(std::cos(2 * M_PI * r1) * static_cast<double>(std::sqrt(1 - z * z)));
+    const float y = static_cast<float>(std::sin(2 * M_PI * r1) * static_cast<double>(std::sqrt(1 - z * z)));
+    return uvw.local(x, y, z);
+  }
+  return vec3(1, 0, 0);
+}
+
+vec3 sky(const ray &r, float scale) {
+  const vec3 d = unit_vector(r.direction());
+  const float t = 0.5f * (d.y() + 1.0f);
+  return scale * ((1.0f - t) * vec3(1, 1, 1) + t * vec3(0.5f, 0.7f, 1.0f));
+}
+
+}  // namespace
+
+ray camera_ray(const camera_with_blur &cam, float s, float t, rng &R) {
+  const vec3 rd = cam.lens_radius_ * in_unit_disk(R);
+  const vec3 offset = cam.u_ * rd.x() + cam.v_ * rd.y();
+  const float time = static_cast<float>(cam.time0 + R.next() * (cam.time1 - cam.time0));
+  return ray(cam.origin_ + offset,
+             cam.lower_left_corner_ + s * cam.horizontal_ + t * cam.vertical_ - cam.origin_ - offset, time);
+}
+
+bool scatter_once(const ray &r, const hit_record &rec, const graph_info &g, const hitable *lights, rng &R,
+                  ray *out) {
+  const material *m = rec.mat_ptr;
+  if (auto *me = dynamic_cast<const metal *>(m)) {
+    const vec3 refl = reflect(unit_vector(r.direction()), rec.normal);
+    *out = ray(rec.point, refl + me->fuzz_ * in_unit_sphere(R), r.time());
+    return dot(out->direction(), rec.normal) > 0;
+  }
+  if (auto *de = dynamic_cast<const dielectric *>(m)) {
+    const vec3 d = r.direction();
+    const float ior = de->ref_idx_;
+    const bool inside = dot(d, rec.normal) > 0;
+    vec3 refracted;
+    const bool ok = refract(d, inside ? -rec.normal : rec.normal,
+                            inside ? ior : static_cast<float>(1.0 / ior), refracted);
+    *out = ray(rec.point, (!ok || R.next() < 0.5) ? reflect(d, rec.normal) : refracted, r.time());
+    return true;
+  }
+  if (!dynamic_cast<const lambertian *>(m)) return false;
+  const onb uvw(rec.normal);
+  if (lights && R.next() < 0.5) {
+    *out = ray(rec.point, light_random(lights, rec.point, R), r.time());
+  } else {
+    const vec3 h = on_hemisphere(R);
+    *out = ray(rec.point, uvw.local(h.x(), h.y(), h.z()), r.time());
+  }
+  (void)g;
+  return true;
+}
+
+vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitable *lights, int depth,
+           const render_opts &o, rng &R) {
+  R.bounce(static_cast<uint32_t>(depth + 1));
+  hit_record rec;
+  uint32_t leaf = 0;
+  if (!hit(world, g, r, 0.001f, kFltMax, rec, 0, &leaf)) {
+    return o.background > 0 ? sky(r, o.background) : vec3(0, 0, 0);
+  }
+  const material *m = rec.mat_ptr;
+  // material::emitted: only diffuse_light emits, one-sided (src/material.cc:79-86).
+  vec3 emitted(0, 0, 0);
+  if (auto *dl = dynamic_cast<const diffuse_light *>(m)) {
+    if (dot(rec.normal, r.direction()) < 0) emitted = tex_value(dl->emit_, rec.u, rec.v, rec.point);
+    return emitted;  // diffuse_light never scatters (headers/material.h:64-67)
+  }
+  if (depth >= o.max_depth) return emitted;
+  if (auto *me = dynamic_cast<const metal *>(m)) {
+    // src/material.cc:88-98.
+    const vec3 refl = reflect(unit_vector(r.direction()), rec.normal);
+    const ray scattered(rec.point, refl + me->fuzz_ * in_unit_sphere(R), r.time());
+    if (!(dot(scattered.direction(), rec.normal) > 0)) return emitted;
+    return me->albedo_ * color(scattered, world, g, lights, depth + 1, o, R);
+  }
+  if (auto *de = dynamic_cast<const dielectric *>(m)) {
+    // src/material.cc:19-70.
+    const vec3 d = r.direction();
+    const vec3 reflected = reflect(d, rec.normal);
+    vec3 outward, refracted;
+    float ni_over_nt, cosine, reflect_prob;
+    const float ior = de->ref_idx_;
+    if (dot(d, rec.normal) > 0) {
+      outward = -rec.normal;
+      ni_over_nt = ior;
+      cosine = dot(d, rec.normal) / d.length();
+      cosine = std::sqrt(1 - ior * ior * (1 - cosine * cosine));
+    } else {
+      outward = rec.normal;
+      ni_over_nt = static_cast<float>(1.0 / ior);
+      cosine = -dot(d, rec.normal) / d.length();
+    }
+    reflect_prob = refract(d, outward, ni_over_nt, refracted) ? schlick(cosine, ior) : 1.0f;
+    const ray scattered(rec.point, R.next() < reflect_prob ? reflected : refracted, r.time());
+    return vec3(1, 1, 1) * color(scattered, world, g, lights, depth + 1, o, R);
+  }
+  auto *lam = dynamic_cast<const lambertian *>(m);
+  if (!lam) return emitted;
+  // src/material.cc:3-17 + src/utils.cc:73-81.
+  const vec3 atten = tex_value(lam->albedo_, rec.u, rec.v, rec.point);
+  const onb uvw(rec.normal);
+  vec3 dir;
+  float pdf;
+  if (lights) {
+    // mixture_pdf(hitable_pdf(lights), cosine_pdf), light first
+    // (headers/rect_box.h:118-131).
+    if (R.next() < 0.5) {
+      dir = light_random(lights, rec.point, R);
+    } else {
+      const vec3 h = on_hemisphere(R);
+      dir = uvw.local(h.x(), h.y(), h.z());
+    }
+    const float pl = light_pdf(lights, g, rec.point, dir);
+    float pc = dot(unit_vector(dir), uvw.w);
+    if (!(pc > 0)) pc = 0;
+    pdf = static_cast<float>(0.5 * pl + 0.5 * pc);
+  } else {
+    // Scene without a light list (defined extension): cosine_pdf alone.
+    const vec3 h = on_hemisphere(R);
+    dir = uvw.local(h.x(), h.y(), h.z());
+    float pc = dot(unit_vector(dir), uvw.w);
+    if (!(pc > 0)) pc = 0;
+    pdf = pc;
+  }
+  const ray scattered(rec.point, dir, r.time());
+  float spdf = dot(rec.normal, unit_vector(scattered.direction()));
+  spdf = spdf < 0 ? 0.0f : static_cast<float>(spdf / M_PI);
+  return emitted + atten * spdf * color(scattered, world, g, lights, depth + 1, o, R) / pdf;
+}
+
+}  // namespace oracle

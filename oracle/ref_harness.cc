@@ -1,0 +1,324 @@
+// ORACLE — test infrastructure only (see oracle.h).
+//
+// Harness linked with the reference's own, unmodified sources
+// (/root/reference/src/*.cc, compiled where they lie by oracle/Makefile) into
+// oracle/_ref/libtpt_ref.so.  It builds scenes with the REFERENCE's classes —
+// either through the shared scene text tiny-path-tracer_amd/host/scenes.inc or
+// through the reference's own factories (cornell_box() etc. in
+// src/utils.cc) — and answers hit queries with the reference's hitable::hit
+// and renders with its color() in main.cpp's thread-per-row loop
+// (main.cpp:109-159).  Used to pin the oracle and as the CPU baseline.
+//
+// Include paths resolve to the reference's headers (-I .../headers).
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "camera.h"
+#include "hitable.h"
+#include "hitable_list.h"
+#include "material.h"
+#include "rect_box.h"
+#include "sphere.h"
+#include "texture.h"
+#include "utils.h"
+
+#define TPT_NEW_PERLIN(scale, octaves) (new perlin_noise_texture((scale)))
+#include "../tiny-path-tracer_amd/host/scenes.inc"
+#undef TPT_NEW_PERLIN
+
+hitable *two_checker_spheres();  // defined in src/utils.cc, not declared in its header
+
+namespace {
+
+// Material stand-in that tags each primitive occurrence with its leaf id
+// (hit_record has no id field; the proxy rides in rec.mat_ptr).
+struct proxy_material : material {
+  int leaf = -1;
+  int mat_index = -1;
+};
+
+struct ref_scene_t {
+  hitable *world = nullptr;   // materials replaced by proxies
+  hitable *render_world = nullptr;  // original materials
+  hitable *lights = nullptr;
+  std::vector<unsigned char> image;
+};
+
+struct tagger {
+  int next_leaf = 0;
+  std::map<material *, int> mat_index;
+  int index_of(material *m) {
+    auto it = mat_index.find(m);
+    if (it != mat_index.end()) return it->second;
+    const int id = static_cast<int>(mat_index.size());
+    mat_index[m] = id;
+    return id;
+  }
+  void tag(material **slot) {
+    auto *p = new proxy_material();
+    p->leaf = next_leaf++;
+    p->mat_index = index_of(*slot);
+    *slot = p;
+  }
+  // Depth-first, left before right, an n==1 node's duplicate child once.
+  void walk(hitable *h) {
+    if (auto *b = dynamic_cast<bvh_node *>(h)) {
+      walk(b->left_);
+      if (b->right_ != b->left_) walk(b->right_);
+    } else if (auto *l = dynamic_cast<hitable_list *>(h)) {
+      for (int i = 0; i < l->list_size_; ++i) walk(l->list_[i]);
+    } else if (auto *bx = dynamic_cast<box *>(h)) {
+      walk(bx->list_ptr_);
+    } else if (auto *f = dynamic_cast<flip_normal *>(h)) {
+      walk(f->ptr_);
+    } else if (auto *t = dynamic_cast<translate *>(h)) {
+      walk(t->ptr_);
+    } else if (auto *r = dynamic_cast<rotate_y *>(h)) {
+      walk(r->ptr_);
+    } else if (auto *s = dynamic_cast<sphere *>(h)) {
+      tag(&s->mat_ptr_);
+    } else if (auto *q = dynamic_cast<xy_rect *>(h)) {
+      tag(&q->mat_ptr_);
+    } else if (auto *q2 = dynamic_cast<xz_rect *>(h)) {
+      tag(&q2->mat_ptr_);
+    } else if (auto *q3 = dynamic_cast<yz_rect *>(h)) {
+      tag(&q3->mat_ptr_);
+    }
+  }
+};
+
+std::vector<unsigned char> noise_image(int width, int height, uint64_t seed) {
+  // Same generator as tiny-path-tracer_amd/host/scenes.cc (splitmix64).
+  std::vector<unsigned char> img(static_cast<size_t>(width) * height * 3);
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  for (size_t i = 0; i < img.size(); i += 8) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    for (size_t k = 0; k < 8 && i + k < img.size(); ++k) img[i + k] = static_cast<unsigned char>(z >> (8 * k));
+  }
+  return img;
+}
+
+bool build(const std::string &name, ref_scene_t *s, hitable **world) {
+  hitable *lights = nullptr;
+  if (name == "cornell555") {
+    *world = tpt_scene_cornell555(&lights);
+  } else if (name == "random_spheres") {
+    *world = tpt_scene_random_spheres(11, 0, nullptr, 0, 0);
+  } else if (name == "random_spheres_textured") {
+    if (s->image.empty()) s->image = noise_image(2048, 1024, 5489);
+    *world = tpt_scene_random_spheres(11, 1, s->image.data(), 2048, 1024);
+  } else if (name.rfind("random_spheres_n", 0) == 0) {
+    const long target = std::strtol(name.c_str() + 16, nullptr, 10);
+    int grid = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(target)) / 2.0));
+    if (grid < 1) grid = 1;
+    *world = tpt_scene_random_spheres(grid, 0, nullptr, 0, 0);
+  } else if (name == "ref_cornell_box") {
+    *world = tpt_scene_ref_cornell_box(&lights);
+  } else if (name == "ref_sphere_cornell_box") {
+    *world = tpt_scene_ref_sphere_cornell_box(&lights);
+  } else if (name == "ref_two_perlin_spheres") {
+    *world = tpt_scene_ref_two_perlin_spheres();
+  } else if (name == "ref_light_spheres") {
+    *world = tpt_scene_ref_light_spheres(&lights);
+  } else if (name == "ref_two_checker_spheres") {
+    *world = tpt_scene_ref_two_checker_spheres();
+  } else if (name == "box_list") {
+    *world = tpt_scene_box_list();
+  } else if (name == "native:cornell_box") {
+    // The reference's own factory and its driver's light list (main.cpp:99-106).
+    *world = cornell_box();
+    hitable **l = new hitable *[2];
+    l[0] = new xz_rect(-100, 100, -150, -50, 298, nullptr);
+    l[1] = new sphere(vec3(120, -50, 40), 120, nullptr);
+    lights = new hitable_list(l, 2);
+  } else if (name == "native:sphere_cornell_box") {
+    *world = sphere_cornell_box();
+    lights = new xz_rect(-150, 150, -150, 150, 300, nullptr);
+  } else if (name == "native:two_perlin_spheres") {
+    *world = two_perlin_spheres();
+  } else if (name == "native:light_spheres") {
+    *world = light_spheres();
+  } else if (name == "native:two_checker_spheres") {
+    *world = two_checker_spheres();
+  } else {
+    return false;
+  }
+  if (!s->lights) s->lights = lights;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Builds `name` twice on fresh threads (drand_r at the default 5489 state):
+// one copy for hit queries (materials tagged) and one for rendering.
+int ref_scene_build(const char *name, void **out) {
+  auto *s = new ref_scene_t();
+  bool ok = true;
+  std::thread t1([&] { ok = build(name, s, &s->world); });
+  t1.join();
+  if (!ok) {
+    delete s;
+    return -1;
+  }
+  std::thread t2([&] { build(name, s, &s->render_world); });
+  t2.join();
+  tagger tg;
+  tg.walk(s->world);
+  *out = s;
+  return 0;
+}
+
+int ref_leaf_count(void *sp) {
+  tagger tg;
+  // count without retagging: walk a fresh build would be costly; count proxies
+  auto *s = static_cast<ref_scene_t *>(sp);
+  struct counter {
+    int n = 0;
+    void walk(hitable *h) {
+      if (auto *b = dynamic_cast<bvh_node *>(h)) {
+        walk(b->left_);
+        if (b->right_ != b->left_) walk(b->right_);
+      } else if (auto *l = dynamic_cast<hitable_list *>(h)) {
+        for (int i = 0; i < l->list_size_; ++i) walk(l->list_[i]);
+      } else if (auto *bx = dynamic_cast<box *>(h)) {
+        walk(bx->list_ptr_);
+      } else if (auto *f = dynamic_cast<flip_normal *>(h)) {
+        walk(f->ptr_);
+      } else if (auto *t = dynamic_cast<translate *>(h)) {
+        walk(t->ptr_);
+      } else if (auto *r = dynamic_cast<rotate_y *>(h)) {
+        walk(r->ptr_);
+      } else {
+        ++n;
+      }
+    }
+  } c;
+  c.walk(s->world);
+  (void)tg;
+  return c.n;
+}
+
+int ref_trace(void *sp, const float *rays, uint64_t n, float tmin, float tmax, int32_t *leaf, float *t, float *rec,
+              int32_t *mat, int nthreads) {
+  auto *s = static_cast<ref_scene_t *>(sp);
+  std::atomic<uint64_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      const uint64_t b = next.fetch_add(1024);
+      if (b >= n) return;
+      const uint64_t e = std::min<uint64_t>(n, b + 1024);
+      for (uint64_t i = b; i < e; ++i) {
+        const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]),
+                    vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+        hit_record hr;
+        if (s->world->hit(r, tmin, tmax, hr)) {
+          auto *p = static_cast<proxy_material *>(hr.mat_ptr);
+          leaf[i] = p->leaf;
+          t[i] = hr.t;
+          if (mat) mat[i] = p->mat_index;
+          if (rec) {
+            const float v[8] = {hr.u, hr.v, hr.point.x(), hr.point.y(), hr.point.z(),
+                                hr.normal.x(), hr.normal.y(), hr.normal.z()};
+            for (int k = 0; k < 8; ++k) rec[k * n + i] = v[k];
+          }
+        } else {
+          leaf[i] = -1;
+          t[i] = 0.0f;
+          if (mat) mat[i] = -1;
+          if (rec)
+            for (int k = 0; k < 8; ++k) rec[k * n + i] = 0.0f;
+        }
+      }
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int k = 1; k < nthreads; ++k) ts.emplace_back(worker);
+  worker();
+  for (auto &th : ts) th.join();
+  return 0;
+}
+
+// params: lookfrom[3], lookat[3], vup[3], vfov, aspect, aperture, focus, t0, t1.
+// out: 24 floats in the order of oracle_camera_fields.
+int ref_camera_fields(const float *p, float *out) {
+  camera cam(vec3(p[0], p[1], p[2]), vec3(p[3], p[4], p[5]), vec3(p[6], p[7], p[8]), p[9], p[10], p[11], p[12],
+             p[13], p[14]);
+  const vec3 *vs[7] = {&cam.origin_, &cam.lower_left_corner_, &cam.horizontal_, &cam.vertical_,
+                       &cam.u_,      &cam.v_,                 &cam.w_};
+  for (int i = 0; i < 7; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = vs[i]->e[j];
+  out[21] = cam.lens_radius_;
+  out[22] = cam.time0;
+  out[23] = cam.time1;
+  return 0;
+}
+
+// Pre-order dump of bvh_node boxes: 6 floats per node.  Returns the node count
+// (at most cap nodes are written).
+int ref_bvh_dump(void *sp, float *out, int cap) {
+  auto *s = static_cast<ref_scene_t *>(sp);
+  int n = 0;
+  struct dumper {
+    float *out;
+    int cap;
+    int *n;
+    void walk(hitable *h) {
+      auto *b = dynamic_cast<bvh_node *>(h);
+      if (!b) return;
+      if (*n < cap)
+        for (int k = 0; k < 3; ++k) {
+          out[6 * *n + k] = b->box_.min_[k];
+          out[6 * *n + 3 + k] = b->box_.max_[k];
+        }
+      ++*n;
+      walk(b->left_);
+      if (b->right_ != b->left_) walk(b->right_);
+    }
+  } d{out, cap, &n};
+  d.walk(s->world);
+  return n;
+}
+
+// The reference's render loop (main.cpp:109-159) on this scene: one
+// std::async task per row, each with a fresh thread_local generator, summing
+// de_nan(color()) per pixel into sum (3·w·h floats, row 0 = bottom).  Camera
+// from params as in ref_camera_fields.  Returns 0, or -1 without a light list.
+int ref_render(void *sp, const float *camp, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
+               float *sum) {
+  auto *s = static_cast<ref_scene_t *>(sp);
+  if (!s->lights) return -1;
+  camera cam(vec3(camp[0], camp[1], camp[2]), vec3(camp[3], camp[4], camp[5]), vec3(camp[6], camp[7], camp[8]),
+             camp[9], camp[10], camp[11], camp[12], camp[13], camp[14]);
+  auto row = [&](int j) {
+    for (uint32_t i = 0; i < w; i++) {
+      vec3 col(0, 0, 0);
+      for (uint32_t k = 0; k < spp; k++) {
+        const float u = ((float)i + drand_r()) / (float)w;
+        const float v = ((float)j + drand_r()) / (float)h;
+        ray r = cam.get_ray(u, v);
+        col += de_nan(color(r, s->render_world, s->lights, 0, static_cast<int>(max_depth)));
+      }
+      for (int c = 0; c < 3; ++c) sum[(static_cast<size_t>(j) * w + i) * 3 + c] = col[c];
+    }
+  };
+  std::vector<std::future<void>> futures;
+  for (int j = static_cast<int>(h) - 1; j >= 0; j--)
+    futures.emplace_back(std::async(std::launch::deferred | std::launch::async, row, j));
+  for (auto &f : futures) f.wait();
+  return 0;
+}
+
+}  // extern "C"

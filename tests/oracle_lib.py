@@ -1,0 +1,159 @@
+"""ctypes bindings to the oracle libraries (test infrastructure).
+
+liboracle.so      oracle/_build — the CPU restatement of the reference path.
+libtpt_ref.so     oracle/_ref   — the reference's own sources + harness; only
+                  present where the reference tree was available at build time.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtpt_ref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+RAYS_PRIMARY, RAYS_SECONDARY, RAYS_UNIFORM, RAYS_ADVERSARIAL = 0, 1, 2, 3
+
+
+def _nthreads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        self.L = L
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_scene_build.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p)]
+        L.oracle_scene_destroy.argtypes = [C.c_void_p]
+        L.oracle_leaf_count.argtypes = [C.c_void_p]
+        L.oracle_leaf_count.restype = C.c_uint32
+        L.oracle_scene_settings.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4 + [C.POINTER(C.c_float)]
+        L.oracle_camera_fields.argtypes = [C.c_void_p, _f32p]
+        L.oracle_trace.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_float, C.c_float, _i32p, _f32p,
+                                   C.c_void_p, C.c_void_p, C.c_int]
+        L.oracle_make_rays.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, _f32p, C.c_int]
+        L.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                    C.c_uint32, C.c_float, C.c_int, C.c_int, _f32p, C.c_void_p]
+
+    def scene(self, name, seed=5489):
+        return OracleScene(self, name, seed)
+
+
+class OracleScene:
+    def __init__(self, lib, name, seed):
+        self.lib = lib
+        self.name = name
+        h = C.c_void_p()
+        if lib.L.oracle_scene_build(name.encode(), seed, C.byref(h)) != 0:
+            raise ValueError(lib.L.oracle_last_error().decode())
+        self.h = h
+        w, hh, spp, d = (C.c_uint32() for _ in range(4))
+        bg = C.c_float()
+        lib.L.oracle_scene_settings(h, C.byref(w), C.byref(hh), C.byref(spp), C.byref(d), C.byref(bg))
+        self.width, self.height, self.spp, self.max_depth = w.value, hh.value, spp.value, d.value
+        self.background = bg.value
+
+    def __del__(self):
+        try:
+            self.lib.L.oracle_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def n_leaves(self):
+        return self.lib.L.oracle_leaf_count(self.h)
+
+    def camera_fields(self):
+        out = np.zeros(24, np.float32)
+        self.lib.L.oracle_camera_fields(self.h, out)
+        return out
+
+    def make_rays(self, kind, n, seed=1, nthreads=None):
+        rays = np.zeros((6, n), np.float32)
+        self.lib.L.oracle_make_rays(self.h, kind, n, seed, rays.reshape(-1), nthreads or _nthreads())
+        return rays
+
+    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None):
+        n = rays.shape[1]
+        rays = np.ascontiguousarray(rays, np.float32)
+        leaf = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        rec = np.zeros((8, n), np.float32) if record else None
+        mat = np.zeros(n, np.int32) if record else None
+        self.lib.L.oracle_trace(self.h, rays.reshape(-1), n, tmin, tmax, leaf, t,
+                                rec.ctypes.data if record else None, mat.ctypes.data if record else None,
+                                nthreads or _nthreads())
+        return (leaf, t, rec, mat) if record else (leaf, t)
+
+    def render(self, width, height, spp, max_depth=None, seed=1, sample_offset=0, background=None,
+               rng_mode=0, nthreads=None):
+        s = np.zeros(width * height * 3, np.float32)
+        sq = np.zeros(width * height * 3, np.float32)
+        self.lib.L.oracle_render(self.h, width, height, spp, max_depth if max_depth is not None else self.max_depth,
+                                 seed, sample_offset, self.background if background is None else background,
+                                 rng_mode, nthreads or _nthreads(), s, sq.ctypes.data)
+        return s.reshape(height, width, 3), sq.reshape(height, width, 3)
+
+
+class Reference:
+    """The reference's own code (oracle/_ref/libtpt_ref.so)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        self.L = L
+        L.ref_scene_build.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.ref_leaf_count.argtypes = [C.c_void_p]
+        L.ref_trace.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_float, C.c_float, _i32p, _f32p,
+                                C.c_void_p, C.c_void_p, C.c_int]
+        L.ref_camera_fields.argtypes = [_f32p, _f32p]
+        L.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
+        L.ref_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+
+    def scene(self, name):
+        h = C.c_void_p()
+        if self.L.ref_scene_build(name.encode(), C.byref(h)) != 0:
+            raise ValueError(f"reference harness: unknown scene {name}")
+        return h
+
+    def leaf_count(self, h):
+        return self.L.ref_leaf_count(h)
+
+    def trace(self, h, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None):
+        n = rays.shape[1]
+        rays = np.ascontiguousarray(rays, np.float32)
+        leaf = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        rec = np.zeros((8, n), np.float32) if record else None
+        mat = np.zeros(n, np.int32) if record else None
+        self.L.ref_trace(h, rays.reshape(-1), n, tmin, tmax, leaf, t, rec.ctypes.data if record else None,
+                         mat.ctypes.data if record else None, nthreads or _nthreads())
+        return (leaf, t, rec, mat) if record else (leaf, t)
+
+    def camera_fields(self, params):
+        out = np.zeros(24, np.float32)
+        self.L.ref_camera_fields(np.asarray(params, np.float32), out)
+        return out
+
+    def bvh_dump(self, h, cap=1 << 20):
+        out = np.zeros(6 * cap, np.float32)
+        n = self.L.ref_bvh_dump(h, out, cap)
+        return out[: 6 * min(n, cap)].reshape(-1, 6)
+
+    def render(self, h, cam_params, width, height, spp, max_depth):
+        s = np.zeros(width * height * 3, np.float32)
+        rc = self.L.ref_render(h, np.asarray(cam_params, np.float32), width, height, spp, max_depth, s)
+        if rc != 0:
+            raise ValueError("reference render needs a light list")
+        return s.reshape(height, width, 3)
