@@ -169,6 +169,9 @@ typedef struct tpt_flat_scene {
   const tpt_perlin *perlin; /* NULL when no Perlin texture */
   const tpt_light *lights;  /* light list flattened; n_lights == 0: no light sampling */
   uint32_t n_lights;
+  uint32_t light_mode; /* 0: the light root is one shape (no pick draw); 1: a hitable_list of
+                          shapes, picked with int(u*n) as hitable_list::random does; 2: nested
+                          lists, picked by cumulative weight */
   uint32_t n_leaves; /* number of leaf ids */
   uint32_t max_depth; /* deepest bvh_node nesting */
 } tpt_flat_scene;
@@ -259,15 +262,22 @@ typedef struct tpt_render_desc {
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* 0: reference estimator; 1: learned proposal mixed in by one-sample MIS */
   uint32_t queue_capacity; /* paths in flight; 0 = automatic */
-  uint32_t flags;          /* reserved, must be 0 */
+  uint32_t flags;          /* TPT_RENDER_PROFILE or 0 */
 } tpt_render_desc;
+
+/* flags: time every trace / shade launch with HIP events (fills the *_ms and
+ * *_launches fields of tpt_render_stats). */
+#define TPT_RENDER_PROFILE 1u
 
 typedef struct tpt_render_stats {
   uint64_t paths;          /* camera paths completed */
   uint64_t segments;       /* closest-hit queries (ray segments) */
   uint64_t light_queries;  /* light-pdf intersections */
   uint64_t nan_samples;    /* samples with a NaN channel zeroed by de_nan */
-  double kernel_ms;        /* summed kernel time of the pass */
+  double kernel_ms;        /* GPU time of the whole pass (events around it) */
+  uint64_t nodes_visited;  /* pre-order slots visited by closest-hit queries */
+  double trace_ms, shade_ms;          /* with TPT_RENDER_PROFILE */
+  uint64_t trace_launches, shade_launches;
 } tpt_render_stats;
 
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,

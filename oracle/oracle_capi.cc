@@ -333,3 +333,83 @@ int oracle_render(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, u
 }
 
 }  // extern "C"
+
+// ---- primitive-level access for closed-form tests --------------------------
+extern "C" {
+
+void oracle_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t *out) {
+  const oracle::philox_out o = oracle::philox4x32_10(c0, c1, c2, c3, k0, k1);
+  for (int i = 0; i < 4; ++i) out[i] = o.v[i];
+}
+
+// AABB::hit on box (lo, hi) for ray (o, d).
+int oracle_slab(const float *lo, const float *hi, const float *o, const float *d, float tmin, float tmax) {
+  return oracle::slab_hit(AABB(vec3(lo[0], lo[1], lo[2]), vec3(hi[0], hi[1], hi[2])),
+                          ray(vec3(o[0], o[1], o[2]), vec3(d[0], d[1], d[2])), tmin, tmax)
+             ? 1
+             : 0;
+}
+
+// sphere::hit / rect hits on a single primitive; rec as in oracle_trace.
+// kind 0: sphere p = (cx, cy, cz, r); kind 1/2/3: xy/xz/yz rect p = (a0, a1, b0, b1, k).
+int oracle_prim_hit(int kind, const float *p, const float *o, const float *d, float tmin, float tmax, float *t,
+                    float *rec) {
+  hitable *h = nullptr;
+  if (kind == 0) h = new sphere(vec3(p[0], p[1], p[2]), p[3], nullptr);
+  if (kind == 1) h = new xy_rect(p[0], p[1], p[2], p[3], p[4], nullptr);
+  if (kind == 2) h = new xz_rect(p[0], p[1], p[2], p[3], p[4], nullptr);
+  if (kind == 3) h = new yz_rect(p[0], p[1], p[2], p[3], p[4], nullptr);
+  if (!h) return -1;
+  oracle::graph_info g;
+  oracle::index_graph(h, &g);
+  hit_record hr;
+  uint32_t leaf;
+  const bool ok = oracle::hit(h, g, ray(vec3(o[0], o[1], o[2]), vec3(d[0], d[1], d[2])), tmin, tmax, hr, 0, &leaf);
+  delete h;
+  if (!ok) return 0;
+  *t = hr.t;
+  const float v[8] = {hr.u, hr.v, hr.point.x(), hr.point.y(), hr.point.z(), hr.normal.x(), hr.normal.y(), hr.normal.z()};
+  for (int k = 0; k < 8; ++k) rec[k] = v[k];
+  return 1;
+}
+
+// The product's camera constructor with explicit parameters (for comparison
+// with the reference's): params as ref_camera_fields.
+int oracle_camera_from_params(const float *p, float *out) {
+  camera_with_blur cam(vec3(p[0], p[1], p[2]), vec3(p[3], p[4], p[5]), vec3(p[6], p[7], p[8]), p[9], p[10], p[11],
+                       p[12], p[13], p[14]);
+  const vec3 *vs[7] = {&cam.origin_, &cam.lower_left_corner_, &cam.horizontal_, &cam.vertical_,
+                       &cam.u_,      &cam.v_,                 &cam.w_};
+  for (int i = 0; i < 7; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = vs[i]->e[j];
+  out[21] = cam.lens_radius_;
+  out[22] = cam.time0;
+  out[23] = cam.time1;
+  return 0;
+}
+
+// Pre-order bvh_node boxes of the scene's world, as ref_bvh_dump.
+int oracle_bvh_dump(const oracle_scene *s, float *out, int cap) {
+  int n = 0;
+  struct dumper {
+    float *out;
+    int cap;
+    int *n;
+    void walk(const hitable *h) {
+      auto *b = dynamic_cast<const bvh_node *>(h);
+      if (!b) return;
+      if (*n < cap)
+        for (int k = 0; k < 3; ++k) {
+          out[6 * *n + k] = b->box_.min_[k];
+          out[6 * *n + 3 + k] = b->box_.max_[k];
+        }
+      ++*n;
+      walk(b->left_);
+      if (b->right_ != b->left_) walk(b->right_);
+    }
+  } d{out, cap, &n};
+  d.walk(s->spec.world);
+  return n;
+}
+
+}  // extern "C"

@@ -44,6 +44,34 @@ class Oracle:
         L.oracle_make_rays.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, _f32p, C.c_int]
         L.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                     C.c_uint32, C.c_float, C.c_int, C.c_int, _f32p, C.c_void_p]
+        L.oracle_philox.argtypes = [C.c_uint32] * 6 + [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")]
+        L.oracle_slab.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float]
+        L.oracle_prim_hit.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_float, C.c_float, _f32p, _f32p]
+        L.oracle_camera_from_params.argtypes = [_f32p, _f32p]
+        L.oracle_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
+
+    def philox(self, ctr, key):
+        out = np.zeros(4, np.uint32)
+        self.L.oracle_philox(*[int(c) for c in ctr], *[int(k) for k in key], out)
+        return out
+
+    def slab(self, lo, hi, o, d, tmin=0.001, tmax=3.4028234663852886e38):
+        f = lambda v: np.asarray(v, np.float32)
+        return bool(self.L.oracle_slab(f(lo), f(hi), f(o), f(d), tmin, tmax))
+
+    def prim_hit(self, kind, params, o, d, tmin=0.001, tmax=3.4028234663852886e38):
+        """kind: 'sphere' (cx,cy,cz,r) or 'xy'/'xz'/'yz' (a0,a1,b0,b1,k).  Returns (t, rec[8]) or None."""
+        k = {"sphere": 0, "xy": 1, "xz": 2, "yz": 3}[kind]
+        t = np.zeros(1, np.float32)
+        rec = np.zeros(8, np.float32)
+        f = lambda v: np.asarray(v, np.float32)
+        ok = self.L.oracle_prim_hit(k, f(params), f(o), f(d), tmin, tmax, t, rec)
+        return (t[0], rec) if ok == 1 else None
+
+    def camera_from_params(self, params):
+        out = np.zeros(24, np.float32)
+        self.L.oracle_camera_from_params(np.asarray(params, np.float32), out)
+        return out
 
     def scene(self, name, seed=5489):
         return OracleScene(self, name, seed)
@@ -77,6 +105,11 @@ class OracleScene:
         out = np.zeros(24, np.float32)
         self.lib.L.oracle_camera_fields(self.h, out)
         return out
+
+    def bvh_dump(self, cap=1 << 20):
+        out = np.zeros(6 * cap, np.float32)
+        n = self.lib.L.oracle_bvh_dump(self.h, out, cap)
+        return out[: 6 * min(n, cap)].reshape(-1, 6)
 
     def make_rays(self, kind, n, seed=1, nthreads=None):
         rays = np.zeros((6, n), np.float32)

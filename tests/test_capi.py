@@ -1,0 +1,121 @@
+"""The C ABI library: loads, exports what include/tpt.h declares, builds and
+flattens scenes on the host, and refuses device work without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "tpt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tpt_[a-z_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(tpt):
+    L = tpt.lib()
+    names = declared_functions()
+    assert "tpt_trace_closest" in names and "tpt_render" in names
+    for n in names:
+        assert hasattr(L, n), f"libtpt.so lacks {n}"
+
+
+def test_abi_version_and_error_path(tpt):
+    L = tpt.lib()
+    assert L.tpt_abi_version() == 1
+    h = C.c_void_p()
+    rc = L.tpt_host_scene_build(b"no_such_scene", 1, C.byref(h))
+    assert rc == -1 and b"unknown scene" in L.tpt_last_error()
+
+
+@pytest.mark.parametrize("name", ["cornell555", "random_spheres", "random_spheres_textured", "ref_cornell_box",
+                                  "ref_light_spheres", "box_list", "random_spheres_n5000",
+                                  "ref_two_perlin_spheres", "ref_sphere_cornell_box"])
+def test_flattened_structure(tpt, oracle, name):
+    hs = tpt.HostScene(name)
+    f = hs.flat
+    slots = hs.slots()
+    assert f.n_leaves == oracle.scene(name).n_leaves
+    # pre-order skip links and right-child links stay inside the subtree
+    for i, (fv, a, b) in enumerate(slots):
+        if b >> 28 == 0:
+            right = b & 0x0FFFFFFF
+            assert i < a <= len(slots)
+            assert right == 0x0FFFFFFF or i + 1 < right < a
+    # every leaf id is used once
+    ids = []
+    for fv, a, b in slots:
+        kind = b >> 28
+        if kind in (1, 2):
+            ids.append(b & 0x0FFFFFFF)
+        elif kind == 3:
+            base = np.float32(fv[0]).view(np.uint32)
+            cnt = np.float32(fv[1]).view(np.uint32)
+            ids.extend(range(int(base), int(base) + int(cnt)))
+    assert sorted(ids) == list(range(f.n_leaves))
+
+
+def test_box_nodes_are_unions_of_children(tpt):
+    """bvh_node box == surrounding_box(children) (src/hitable.cc:50-55)."""
+    hs = tpt.HostScene("random_spheres")
+    slots = hs.slots()
+
+    def box_of(i):
+        fv, a, b = slots[i]
+        kind = b >> 28
+        if kind == 0:
+            return np.array(fv[:3], np.float32), np.array(fv[3:6], np.float32)
+        assert kind == 1
+        c, r = np.array(fv[:3], np.float32), np.float32(fv[3])
+        return c - r, c + r
+
+    for i, (fv, a, b) in enumerate(slots):
+        if b >> 28 != 0:
+            continue
+        right = b & 0x0FFFFFFF
+        lo, hi = box_of(i + 1)
+        if right != 0x0FFFFFFF:
+            lo2, hi2 = box_of(right)
+            lo, hi = np.where(lo < lo2, lo, lo2), np.where(hi > hi2, hi, hi2)
+        assert np.array_equal(np.array(fv[:3], np.float32), lo)
+        assert np.array_equal(np.array(fv[3:6], np.float32), hi)
+
+
+def test_camera_matches_oracle(tpt, oracle):
+    for name in ("cornell555", "random_spheres", "ref_cornell_box"):
+        a = tpt.HostScene(name).camera.as_array()
+        b = oracle.scene(name).camera_fields()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_unsupported_types_fail_cleanly(tpt):
+    # the product scene library has no unsupported scene; the flattener's
+    # refusal is exercised through a scene name that does not exist
+    with pytest.raises(tpt.TptError):
+        tpt.HostScene("random_spheres_n0")
+
+
+def test_device_calls_refuse_without_gpu(tpt):
+    L = tpt.lib()
+    if L.tpt_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    hs = tpt.HostScene("cornell555")
+    h = C.c_void_p()
+    rc = L.tpt_scene_create(C.byref(hs.flat), C.byref(h))
+    assert rc == -5 and b"gfx950" in L.tpt_last_error()
+    with pytest.raises(tpt.TptError):
+        tpt.Scene(hs)
+
+
+def test_tonemap_matches_reference_transform(tpt):
+    img = np.array([[[0.0, 0.25, 1.5], [np.nan, 1.0, 0.01]]], np.float32)
+    txt = tpt.tonemap_ppm(img)
+    lines = txt.split("\n")
+    assert lines[0] == "P3" and lines[1] == "2 1"
+    # sqrt(0.25) = 0.5 -> int(255.99*0.5) = 127; 1.5 -> clamp 255; sqrt(.01)=.1 -> 25
+    assert lines[3].split() == ["0", "127", "255"]
+    assert lines[4].split() == ["0", "255", "25"]

@@ -1,0 +1,93 @@
+"""tpt_render against the oracle's color() and the reference's own render.
+
+1. Same Philox streams: the oracle and the device draw the same numbers per
+   (pixel, sample, bounce), so most pixels agree to float rounding; the rest
+   differ where a last-bit difference (device sin/cos/atan2, iterative vs
+   recursive accumulation) flips a branch.
+2. Independent streams: every pixel mean agrees within a few sigma.
+3. The reference's own color() + thread-per-row loop (oracle/_ref), per pixel.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def cam_for(tpt, name):
+    return tpt.HostScene(name).camera
+
+
+def gpu_render(tpt, name, w, h, spp, seed, max_depth=None):
+    sc = tpt.Scene.named(name)
+    s, sq, st = sc.render(width=w, height=h, spp=spp, seed=seed, max_depth=max_depth, camera=sc.host.camera)
+    return s.cpu().numpy(), sq.cpu().numpy(), st
+
+
+def zscores(s1, q1, n1, s2, q2, n2):
+    m1, m2 = s1 / n1, s2 / n2
+    v1 = np.maximum(q1 / n1 - m1 * m1, 0) / n1
+    v2 = np.maximum(q2 / n2 - m2 * m2, 0) / n2
+    return (m1 - m2) / np.sqrt(v1 + v2 + 1e-12)
+
+
+@pytest.mark.parametrize("name,w,spp", [("cornell555", 48, 32), ("random_spheres", 48, 16), ("ref_cornell_box", 40, 16),
+                                        ("random_spheres_textured", 40, 8), ("box_list", 40, 8)])
+def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
+    s_g, q_g, st = gpu_render(tpt, name, w, w, spp, seed=11)
+    s_c, q_c = oracle.scene(name).render(w, w, spp, seed=11)
+    assert st["paths"] == w * w * spp
+    close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
+    frac = close.all(axis=2).mean()
+    print(f"{name}: {frac:.4f} of pixels equal to rounding; mean gpu {s_g.mean():.5f} cpu {s_c.mean():.5f}")
+    assert frac > 0.9
+    z = zscores(s_g, q_g, spp, s_c, q_c, spp)
+    assert np.mean(np.abs(z) > 4) < 0.01
+
+
+@pytest.mark.parametrize("name,w,spp", [("cornell555", 40, 64), ("random_spheres", 40, 32)])
+def test_independent_streams_within_3_sigma(tpt, oracle, device, name, w, spp):
+    s_g, q_g, _ = gpu_render(tpt, name, w, w, spp, seed=123)
+    s_c, q_c = oracle.scene(name).render(w, w, spp, seed=456)
+    z = zscores(s_g, q_g, spp, s_c, q_c, spp)
+    frac3 = np.mean(np.abs(z) > 3)
+    print(f"{name}: |z|>3 in {frac3:.4f} of pixel channels")
+    assert frac3 < 0.015
+    assert abs(s_g.mean() - s_c.mean()) <= 4 * np.sqrt((q_g.mean() / spp) / (w * w * 3 * spp)) + 1e-6
+
+
+def test_against_reference_render(tpt, device, reference):
+    """The reference's own color() with its thread-local mt19937 streams."""
+    name, w, spp = "cornell555", 40, 64
+    s_g, q_g, _ = gpu_render(tpt, name, w, w, spp, seed=99)
+    h = reference.scene(name)
+    camp = [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0]
+    s_r = reference.render(h, camp, w, w, spp, 50)
+    m_g, m_r = s_g / spp, s_r / spp
+    var = np.maximum(q_g / spp - m_g * m_g, 0) / spp
+    z = (m_g - m_r) / np.sqrt(2 * var + 1e-12)
+    frac3 = np.mean(np.abs(z) > 3)
+    print(f"reference render: |z|>3 in {frac3:.4f}; means {m_g.mean():.5f} vs {m_r.mean():.5f}")
+    assert frac3 < 0.02
+    assert abs(m_g.mean() - m_r.mean()) < 0.03 * m_r.mean()
+
+
+def test_progressive_passes_accumulate(tpt, device):
+    """Two passes over sample ranges [0,8) and [8,16) equal one pass of 16."""
+    import torch
+
+    sc = tpt.Scene.named("cornell555")
+    cam = sc.host.camera
+    a, qa, _ = sc.render(width=32, height=32, spp=16, seed=5, camera=cam)
+    b = torch.zeros_like(a)
+    qb = torch.zeros_like(a)
+    sc.render(width=32, height=32, spp=8, seed=5, camera=cam, sum_buf=b, sumsq_buf=qb)
+    sc.render(width=32, height=32, spp=8, seed=5, sample_offset=8, camera=cam, sum_buf=b, sumsq_buf=qb)
+    assert torch.allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+def test_queue_capacity_does_not_change_result(tpt, device):
+    sc = tpt.Scene.named("random_spheres")
+    cam = sc.host.camera
+    a, _, _ = sc.render(width=40, height=40, spp=4, seed=3, camera=cam)
+    b, _, _ = sc.render(width=40, height=40, spp=4, seed=3, camera=cam, queue_capacity=333)
+    assert bool((a == b).all())

@@ -1,0 +1,125 @@
+"""Closed-form, special-case and brute-force checks of the oracle's hit code
+(independent of the reference build)."""
+import numpy as np
+import pytest
+
+INF = np.float32(np.inf)
+FMAX = 3.4028234663852886e38
+
+
+def test_slab_basic(oracle):
+    lo, hi = (0, 0, 0), (1, 1, 1)
+    assert oracle.slab(lo, hi, (0.5, 0.5, -1), (0, 0, 1))
+    assert not oracle.slab(lo, hi, (2.0, 0.5, -1), (0, 0, 1))  # parallel, outside the x slab
+    assert oracle.slab(lo, hi, (0.5, 0.5, -1), (0, 0, -1)) is False  # behind the origin
+    assert oracle.slab(lo, hi, (0.5, 0.5, 2), (0, 0, -1))
+
+
+def test_slab_signed_zero_and_nan(oracle):
+    lo, hi = (0, 0, 0), (1, 1, 1)
+    # d.x = -0.0: 1/d = -inf, slabs swapped; inside the x slab -> still a hit
+    assert oracle.slab(lo, hi, (0.5, 0.5, -1), (-0.0, 0, 1))
+    # origin exactly on the x = 0 face with d.x = 0: (0 - 0) * inf = NaN, which
+    # the reference's ternaries ignore -> accepted
+    assert oracle.slab(lo, hi, (0.0, 0.5, -1), (0.0, 0, 1))
+    assert oracle.slab(lo, hi, (1.0, 0.5, -1), (-0.0, 0, 1))
+    # t_max = NaN never rejects (hitable_list can pass a NaN closest t)
+    assert oracle.slab(lo, hi, (5, 5, 5), (1, 0, 0), 0.001, float("nan"))
+
+
+def test_sphere_closed_forms(oracle):
+    unit = (0, 0, 0, 1)
+    t, rec = oracle.prim_hit("sphere", unit, (0, 0, -5), (0, 0, 1))
+    assert t == 4.0
+    assert np.allclose(rec[2:5], (0, 0, -1)) and np.allclose(rec[5:8], (0, 0, -1))
+    t, _ = oracle.prim_hit("sphere", unit, (0, 0, -5), (0, 0, 2))  # unnormalised direction
+    assert t == 2.0
+    t, _ = oracle.prim_hit("sphere", unit, (0, 0, 0), (1, 0, 0))  # inside: far root
+    assert t == 1.0
+    assert oracle.prim_hit("sphere", unit, (0, 1, -5), (0, 0, 1)) is None  # tangent: disc == 0 is a miss
+    # t_max is exclusive for spheres
+    assert oracle.prim_hit("sphere", unit, (0, 0, -5), (0, 0, 1), tmax=4.0) is None
+    # uv at the -x pole: atan2(0, -1) = pi -> u = 1.0; v = 0.5
+    _, rec = oracle.prim_hit("sphere", unit, (-5, 0, 0), (1, 0, 0))
+    assert abs(rec[0] - 1.0) < 1e-6 and abs(rec[1] - 0.5) < 1e-6
+
+
+def test_sphere_random_against_float64(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        c = rng.uniform(-5, 5, 3)
+        r = rng.uniform(0.2, 3)
+        o = rng.uniform(-20, 20, 3)
+        d = (c + rng.normal(0, r * 0.5, 3)) - o
+        res = oracle.prim_hit("sphere", (*c, r), o, d)
+        oc = o - c
+        a, b, cc = d @ d, 2 * d @ oc, oc @ oc - r * r
+        disc = b * b - 4 * a * cc
+        if disc <= 1e-6 * b * b or res is None:
+            continue
+        t_exact = (-b - np.sqrt(disc)) / (2 * a)
+        if t_exact <= 0.001:
+            t_exact = (-b + np.sqrt(disc)) / (2 * a)
+        assert abs(res[0] - t_exact) <= 1e-4 * abs(t_exact) + 1e-4
+
+
+def test_rect_closed_forms(oracle):
+    q = (0, 1, 0, 1, 1)  # xy rect [0,1]^2 at z = 1
+    t, rec = oracle.prim_hit("xy", q, (0.5, 0.25, 0), (0, 0, 1))
+    assert t == 1.0 and rec[0] == 0.5 and rec[1] == 0.25
+    assert tuple(rec[5:8]) == (0, 0, 1)
+    assert oracle.prim_hit("xy", q, (0, 0, 0), (1, 1, 1)) is not None  # corner: bounds inclusive
+    assert oracle.prim_hit("xy", q, (0, 0, 0), (1.0001, 1, 1)) is None
+    assert oracle.prim_hit("xy", q, (0.5, 0.5, 0), (0, 0, 1), tmax=1.0) is not None  # t == t_max accepted
+    # in-plane ray with the origin on the plane: t = 0/0 = NaN is accepted
+    res = oracle.prim_hit("xy", q, (5, 5, 1), (1, 0, 0))
+    assert res is not None and np.isnan(res[0])
+    t, rec = oracle.prim_hit("xz", (0, 2, 0, 2, 3), (1, 0, 1), (0, 1, 0))
+    assert t == 3.0 and tuple(rec[5:8]) == (0, 1, 0)
+    t, rec = oracle.prim_hit("yz", (0, 2, 0, 2, -3), (0, 1, 1), (-1, 0, 0))
+    assert t == 3.0 and tuple(rec[5:8]) == (1, 0, 0)
+
+
+def _sphere_leaves(tpt, name):
+    hs = tpt.HostScene(name)
+    cs, ids = [], []
+    for f, a, b in hs.slots():
+        if (b >> 28) == 1:
+            cs.append(f[:4])
+            ids.append(b & 0x0FFFFFFF)
+    return np.array(cs, np.float64), np.array(ids)
+
+
+def test_bvh_closest_equals_brute_force(oracle, tpt):
+    """bvh_node::hit returns the closest sphere: against a float64 scan."""
+    s = oracle.scene("random_spheres")
+    spheres, ids = _sphere_leaves(tpt, "random_spheres")
+    assert len(ids) == s.n_leaves
+    rays = np.concatenate([s.make_rays(k, 2000, seed=5) for k in (0, 1, 2)], axis=1)
+    leaf, t = s.trace(rays)
+    o = rays[:3].T.astype(np.float64)
+    d = rays[3:].T.astype(np.float64)
+    oc = o[:, None, :] - spheres[None, :, :3]
+    a = np.einsum("ij,ij->i", d, d)[:, None]
+    b = 2 * np.einsum("ij,ikj->ik", d, oc)
+    c = np.einsum("ikj,ikj->ik", oc, oc) - spheres[None, :, 3] ** 2
+    disc = b * b - 4 * a * c
+    sq = np.sqrt(np.maximum(disc, 0))
+    t0 = (-b - sq) / (2 * a)
+    t1 = (-b + sq) / (2 * a)
+    tt = np.where(t0 > 1e-3, t0, np.where(t1 > 1e-3, t1, np.inf))
+    tt = np.where(disc > 0, tt, np.inf)
+    order = np.sort(tt, axis=1)
+    best = np.argmin(tt, axis=1)
+    with np.errstate(invalid="ignore"):
+        clear = (order[:, 1] - order[:, 0] > 1e-3 * np.abs(order[:, 0])) | ~np.isfinite(order[:, 1])
+    glance = np.abs(disc[np.arange(len(best)), best]) > 1e-3 * b[np.arange(len(best)), best] ** 2
+    sel = clear & glance
+    assert sel.mean() > 0.8
+    want = np.where(np.isfinite(order[:, 0]), ids[best], -1)
+    assert np.array_equal(leaf[sel], want[sel])
+    # t against float64, except on the r = 1000 ground sphere where the
+    # reference's float c = |oc|^2 - r^2 cancels catastrophically
+    ground = ids[np.argmax(spheres[:, 3])]
+    hit = sel & (leaf >= 0) & (leaf != ground)
+    assert np.allclose(t[hit], order[hit, 0], rtol=1e-4)

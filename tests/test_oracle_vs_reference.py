@@ -1,0 +1,33 @@
+"""The oracle against the reference's own code compiled from its sources
+(oracle/_ref), on fresh seeded rays: identical hits, bit for bit."""
+import numpy as np
+import pytest
+
+PAIRS = [("cornell555", "cornell555"), ("random_spheres", "random_spheres"),
+         ("random_spheres_textured", "random_spheres_textured"),
+         ("ref_cornell_box", "native:cornell_box"), ("ref_sphere_cornell_box", "native:sphere_cornell_box"),
+         ("ref_two_checker_spheres", "native:two_checker_spheres"), ("box_list", "box_list"),
+         ("random_spheres_n3000", "random_spheres_n3000")]
+
+
+@pytest.mark.parametrize("mine,ref", PAIRS)
+def test_oracle_equals_reference(oracle, reference, mine, ref):
+    s = oracle.scene(mine)
+    h = reference.scene(ref)
+    assert s.n_leaves == reference.leaf_count(h)
+    for kind in range(4):
+        rays = s.make_rays(kind, 1 << 13, seed=99 + kind)
+        a = s.trace(rays, record=True)
+        b = reference.trace(h, rays, record=True)
+        assert np.array_equal(a[0], b[0]), f"leaf ids differ (kind {kind})"
+        assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)), f"t differs (kind {kind})"
+        assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32)), f"record differs (kind {kind})"
+        assert np.array_equal(a[3], b[3])
+
+
+def test_reference_render_runs(reference):
+    # the reference's own color() + thread-per-row loop on the benchmark scene
+    h = reference.scene("cornell555")
+    cam = [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0]
+    img = reference.render(h, cam, 16, 16, 4, 50)
+    assert np.isfinite(img).all() and img.sum() > 0

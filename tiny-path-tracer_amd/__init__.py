@@ -1,0 +1,265 @@
+"""tiny-path-tracer_amd — MI355X wavefront path tracer, Python host side.
+
+Thin ctypes layer over libtpt.so (the C ABI declared in include/tpt.h).  The
+scene graph is built by the library's C++ scene API (the reference's
+hitable / material / texture / camera classes); Python selects a named scene,
+uploads it and drives the two device entry points that replace the
+reference's per-ray interfaces:
+
+    Scene.trace(rays)   -> tpt_trace_closest   (hitable::hit, headers/hitable.h:32-33)
+    Scene.render(...)   -> tpt_render          (color() + main.cpp's sample loop)
+
+Device buffers are torch tensors on cuda:0 (torch is plumbing only: memory and
+streams).  There is no CPU fallback: without a gfx950 device or without the
+built library every device call raises.
+
+Import as ``importlib.import_module("tiny-path-tracer_amd")``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["lib", "HostScene", "Scene", "TptError", "scene_names", "build", "tonemap_ppm"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtpt.so")
+
+TPT_OK = 0
+
+
+class TptError(RuntimeError):
+    pass
+
+
+class _Slot(C.Structure):
+    _fields_ = [("f", C.c_float * 6), ("a", C.c_uint32), ("b", C.c_uint32)]
+
+
+class FlatScene(C.Structure):
+    _fields_ = [
+        ("slots", C.c_void_p), ("n_slots", C.c_uint32),
+        ("segments", C.c_void_p), ("n_segments", C.c_uint32),
+        ("entries", C.c_void_p), ("n_entries", C.c_uint32),
+        ("xforms", C.c_void_p), ("n_xforms", C.c_uint32),
+        ("materials", C.c_void_p), ("n_materials", C.c_uint32),
+        ("textures", C.c_void_p), ("n_textures", C.c_uint32),
+        ("texels", C.c_void_p), ("n_texels", C.c_uint64),
+        ("perlin", C.c_void_p),
+        ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("light_mode", C.c_uint32),
+        ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3), ("lower_left", C.c_float * 3), ("horizontal", C.c_float * 3),
+        ("vertical", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("w", C.c_float * 3),
+        ("lens_radius", C.c_float), ("time0", C.c_float), ("time1", C.c_float),
+    ]
+
+    def as_array(self):
+        vals = []
+        for name in ("origin", "lower_left", "horizontal", "vertical", "u", "v", "w"):
+            vals.extend(getattr(self, name))
+        vals += [self.lens_radius, self.time0, self.time1]
+        return np.array(vals, np.float32)
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("background", C.c_float), ("n_leaves", C.c_uint32)]
+
+
+class RenderDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("seed", C.c_uint64), ("sample_offset", C.c_uint32), ("background", C.c_float),
+                ("guiding", C.c_uint32), ("queue_capacity", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("segments", C.c_uint64), ("light_queries", C.c_uint64),
+                ("nan_samples", C.c_uint64), ("kernel_ms", C.c_double), ("nodes_visited", C.c_uint64),
+                ("trace_ms", C.c_double), ("shade_ms", C.c_double), ("trace_launches", C.c_uint64),
+                ("shade_launches", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libtpt.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TptError(f"{LIB_PATH} not built: run `make -C tiny-path-tracer_amd` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, f32, i32p = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_void_p
+    L.tpt_abi_version.restype = C.c_int
+    L.tpt_last_error.restype = C.c_char_p
+    L.tpt_scene_names.restype = C.c_char_p
+    L.tpt_device_count.restype = C.c_int
+    L.tpt_host_scene_build.argtypes = [C.c_char_p, u32, C.POINTER(vp)]
+    L.tpt_host_scene_flat.argtypes = [vp, C.POINTER(FlatScene)]
+    L.tpt_host_scene_camera.argtypes = [vp, C.POINTER(Camera)]
+    L.tpt_host_scene_info.argtypes = [vp, C.POINTER(SceneInfo)]
+    L.tpt_host_scene_destroy.argtypes = [vp]
+    L.tpt_scene_create.argtypes = [C.POINTER(FlatScene), C.POINTER(vp)]
+    L.tpt_scene_destroy.argtypes = [vp]
+    L.tpt_trace_closest.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp]
+    L.tpt_trace_closest_record.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp, i32p, vp]
+    L.tpt_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp, C.POINTER(RenderStats), vp]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != TPT_OK:
+        raise TptError(f"{what} failed ({rc}): {lib().tpt_last_error().decode()}")
+
+
+def scene_names():
+    return lib().tpt_scene_names().decode().split(",")
+
+
+class HostScene:
+    """A named scene built and flattened on the host (no device needed)."""
+
+    def __init__(self, name, seed=5489):
+        L = lib()
+        self.name = name
+        h = C.c_void_p()
+        _check(L.tpt_host_scene_build(name.encode(), seed, C.byref(h)), f"tpt_host_scene_build({name})")
+        self.h = h
+        self.flat = FlatScene()
+        _check(L.tpt_host_scene_flat(h, C.byref(self.flat)), "tpt_host_scene_flat")
+        self.camera = Camera()
+        _check(L.tpt_host_scene_camera(h, C.byref(self.camera)), "tpt_host_scene_camera")
+        self.info = SceneInfo()
+        _check(L.tpt_host_scene_info(h, C.byref(self.info)), "tpt_host_scene_info")
+
+    def slots(self):
+        n = self.flat.n_slots
+        arr = (_Slot * n).from_address(self.flat.slots) if n else []
+        return [(tuple(s.f), s.a, s.b) for s in arr]
+
+    def __del__(self):
+        try:
+            lib().tpt_host_scene_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class Scene:
+    """A scene uploaded to the current HIP device."""
+
+    def __init__(self, host_scene):
+        L = lib()
+        if L.tpt_device_count() < 1:
+            raise TptError("no gfx950 device visible: the device path has no CPU fallback")
+        self.host = host_scene
+        h = C.c_void_p()
+        _check(L.tpt_scene_create(C.byref(host_scene.flat), C.byref(h)), "tpt_scene_create")
+        self.h = h
+
+    @classmethod
+    def named(cls, name, seed=5489):
+        return cls(HostScene(name, seed))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tpt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, stream=None):
+        """rays: float32 tensor (6, n) on the device (ox, oy, oz, dx, dy, dz).
+        Returns (leaf int32[n], t float32[n]) [+ (rec float32[8, n], mat int32[n])]."""
+        torch = _torch()
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.shape[0] == 6
+        rays = rays.contiguous()
+        n = rays.shape[1]
+        leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
+        t = torch.empty(n, dtype=torch.float32, device=rays.device)
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device).cuda_stream
+        if record:
+            rec = torch.empty((8, n), dtype=torch.float32, device=rays.device)
+            mat = torch.empty(n, dtype=torch.int32, device=rays.device)
+            _check(lib().tpt_trace_closest_record(self.h, rays.data_ptr(), n, tmin, tmax, leaf.data_ptr(),
+                                                  t.data_ptr(), rec.data_ptr(), mat.data_ptr(), s),
+                   "tpt_trace_closest_record")
+            return leaf, t, rec, mat
+        _check(lib().tpt_trace_closest(self.h, rays.data_ptr(), n, tmin, tmax, leaf.data_ptr(), t.data_ptr(), s),
+               "tpt_trace_closest")
+        return leaf, t
+
+    def render(self, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0, background=None,
+               guiding=0, queue_capacity=0, sum_buf=None, sumsq_buf=None, stream=None, camera=None,
+               profile=False):
+        """Renders into (and returns) device tensors sum, sumsq of shape (H, W, 3);
+        row 0 is the bottom image row as in the reference."""
+        torch = _torch()
+        info = self.host.info
+        desc = RenderDesc()
+        desc.width = width or info.width
+        desc.height = height or info.height
+        desc.spp = info.spp if spp is None else spp
+        desc.max_depth = info.max_depth if max_depth is None else max_depth
+        desc.seed = seed
+        desc.sample_offset = sample_offset
+        desc.background = info.background if background is None else background
+        desc.guiding = guiding
+        desc.queue_capacity = queue_capacity
+        desc.flags = 1 if profile else 0
+        shape = (desc.height, desc.width, 3)
+        if sum_buf is None:
+            sum_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        if sumsq_buf is None:
+            sumsq_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        stats = RenderStats()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        cam = camera if camera is not None else self.host.camera
+        if (width and width != info.width) or (height and height != info.height):
+            if camera is None:
+                raise TptError("a different resolution needs its own camera (aspect ratio)")
+        _check(lib().tpt_render(self.h, C.byref(cam), C.byref(desc), sum_buf.data_ptr(), sumsq_buf.data_ptr(),
+                                C.byref(stats), s), "tpt_render")
+        return sum_buf, sumsq_buf, stats.as_dict()
+
+
+def tonemap_ppm(mean_rgb):
+    """The reference's output transform (main.cpp:135-142, 176-189): sqrt,
+    int(255.99f·c), clamp [0,255], rows written top first.  mean_rgb: float
+    array (H, W, 3) with row 0 at the bottom.  Returns the P3 text."""
+    c = np.sqrt(mean_rgb.astype(np.float32))
+    q = (np.float32(255.99) * c)
+    q = np.where(np.isfinite(q), q, np.float32(-1.0))
+    iv = np.clip(np.trunc(q), 0, 255).astype(np.int32)
+    h, w, _ = iv.shape
+    lines = [f"P3\n{w} {h}\n255\n"]
+    for j in range(h - 1, -1, -1):
+        row = iv[j]
+        lines.append("".join(f"{r} {g} {b} \n" for r, g, b in row))
+    return "".join(lines)
+
+
+def build():
+    """Compiles libtpt.so in-tree (hipcc for gfx950 + g++)."""
+    import subprocess
+
+    subprocess.run(["make", "-C", _HERE, "-j8"], check=True)

@@ -1,0 +1,371 @@
+// Closest-hit traversal on the flattened scene, bit-exact with the
+// reference's recursive hitable::hit.
+//
+// How the reference's semantics map onto the flat layout (include/tpt.h):
+//  * bvh_node::hit (src/hitable.cc:63-90) tests its box with the caller's
+//    (t_min, t_max), then both children with the SAME t_max, and keeps the
+//    left record only if strictly closer.  The set of boxes tested never
+//    depends on hits found, so a pre-order walk with skip links (box miss ->
+//    jump past the subtree) visits exactly the reference's nodes in its
+//    order, with no stack.  For non-NaN t the nested "left if strictly
+//    closer" combines equal "last visited among the minima", i.e. replace on
+//    t <= best.  A NaN t (an in-plane ray on a rect: 0/0, accepted by the
+//    reference's `t > t_max || t < t_min` test) makes the combine depend on
+//    the tree shape, so a segment that produced one is re-evaluated by
+//    segment_exact(), which replays the recursion with an explicit stack.
+//  * AABB::hit (src/aabb.cc:3-19): 1.0f/d per axis (precomputed once per ray,
+//    same bits), swap iff inv < 0, NaN-ignoring ternaries.  No early exit is
+//    needed: t_min only grows and t_max only shrinks, so the final test equals
+//    the per-axis one.
+//  * sphere::hit (src/sphere.cc:13-43): float a, b, c and discriminant; roots
+//    in FP64 ((double)-b -+ sqrt(disc)) / (double)(2a), rounded to float.
+//  * rect hit (src/rect_box.cc:8-91): t = (k - o_axis) / d_axis (IEEE
+//    division), bounds inclusive.
+//  * hitable_list (src/hitable_list.cc:38-51), box, translate, rotate_y,
+//    flip_normal: a LIST leaf holds its primitives in evaluation order with
+//    their wrapper chains; the list's t_max shrinks to each accepted t.
+#ifndef TPT_CSRC_TRAVERSE_H
+#define TPT_CSRC_TRAVERSE_H
+
+#include "dev_common.h"
+
+#pragma clang fp contract(off)
+
+namespace tptd {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+struct hit_t {
+  float t;
+  uint32_t slot;   // slot index of the winning leaf
+  uint32_t entry;  // entry index for LIST leaves, kNone otherwise
+  uint32_t mat;    // material word (material | axis<<28 | flip<<30)
+};
+
+struct slot_t {
+  float f[6];
+  uint32_t a, b;
+};
+
+__device__ __forceinline__ slot_t load_slot(const tpt_slot *p) {
+  const float4 *q = reinterpret_cast<const float4 *>(p);
+  const float4 x = q[0], y = q[1];
+  slot_t s;
+  s.f[0] = x.x, s.f[1] = x.y, s.f[2] = x.z, s.f[3] = x.w, s.f[4] = y.x, s.f[5] = y.y;
+  s.a = __float_as_uint(y.z);
+  s.b = __float_as_uint(y.w);
+  return s;
+}
+
+__device__ __forceinline__ bool slab(const slot_t &s, v3 o, v3 inv, float tmin, float tmax) {
+  float t0 = (s.f[0] - o.x) * inv.x, t1 = (s.f[3] - o.x) * inv.x;
+  if (inv.x < 0.0f) {
+    const float k = t0;
+    t0 = t1;
+    t1 = k;
+  }
+  tmin = t0 > tmin ? t0 : tmin;
+  tmax = t1 < tmax ? t1 : tmax;
+  t0 = (s.f[1] - o.y) * inv.y, t1 = (s.f[4] - o.y) * inv.y;
+  if (inv.y < 0.0f) {
+    const float k = t0;
+    t0 = t1;
+    t1 = k;
+  }
+  tmin = t0 > tmin ? t0 : tmin;
+  tmax = t1 < tmax ? t1 : tmax;
+  t0 = (s.f[2] - o.z) * inv.z, t1 = (s.f[5] - o.z) * inv.z;
+  if (inv.z < 0.0f) {
+    const float k = t0;
+    t0 = t1;
+    t1 = k;
+  }
+  tmin = t0 > tmin ? t0 : tmin;
+  tmax = t1 < tmax ? t1 : tmax;
+  return !(tmin > tmax);
+}
+
+__device__ __forceinline__ bool sphere_t(float cx, float cy, float cz, float r, v3 o, v3 d, float tmin,
+                                         float tmax, float *t) {
+  const float ox = o.x - cx, oy = o.y - cy, oz = o.z - cz;
+  const float a = d.x * d.x + d.y * d.y + d.z * d.z;
+  const float b = 2.0f * (d.x * ox + d.y * oy + d.z * oz);  // == (float)(2.0 * (double)dot), exact scaling
+  const float c = (ox * ox + oy * oy + oz * oz) - r * r;
+  const float disc = b * b - (4.0f * a) * c;
+  if (disc > 0.0f) {
+    const double sq = __builtin_sqrt(static_cast<double>(disc));
+    const double den = static_cast<double>(2.0f * a);
+    const double nb = static_cast<double>(-b);
+    const float t0 = static_cast<float>((nb - sq) / den);
+    if (t0 < tmax && t0 > tmin) {
+      *t = t0;
+      return true;
+    }
+    const float t1 = static_cast<float>((nb + sq) / den);
+    if (t1 < tmax && t1 > tmin) {
+      *t = t1;
+      return true;
+    }
+  }
+  return false;
+}
+
+// f = (k, a0, a1, b0, b1); axis = normal axis; in-plane axes (i0, i1) are
+// (1,2) for yz, (0,2) for xz, (0,1) for xy.
+__device__ __forceinline__ bool rect_t(const float *f, uint32_t axis, v3 o, v3 d, float tmin, float tmax,
+                                       float *t) {
+  const float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
+  const float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+  const float tt = (f[0] - oa) / da;
+  if (tt > tmax || tt < tmin) return false;
+  const float o0 = axis == 0 ? o.y : o.x, d0 = axis == 0 ? d.y : d.x;
+  const float o1 = axis == 2 ? o.y : o.z, d1 = axis == 2 ? d.y : d.z;
+  const float x = o0 + tt * d0;
+  const float y = o1 + tt * d1;
+  if (x < f[1] || x > f[2] || y < f[3] || y > f[4]) return false;
+  *t = tt;
+  return true;
+}
+
+// Ray in the frame of a wrapper chain (outermost op first), as translate::hit
+// and rotate_y::hit build it (headers/rect_box.h:87-95, src/rect_box.cc:171-180).
+__device__ __forceinline__ void to_local(const tpt_xform &x, v3 *o, v3 *d) {
+  for (uint32_t i = 0; i < x.nops; ++i) {
+    const uint32_t kind = (x.kinds >> (2 * i)) & 3u;
+    if (kind == TPT_XFORM_TRANSLATE) {
+      *o = v3{o->x - x.p[i][0], o->y - x.p[i][1], o->z - x.p[i][2]};
+    } else {
+      const float s = x.p[i][0], c = x.p[i][1];
+      *o = v3{c * o->x - s * o->z, o->y, s * o->x + c * o->z};
+      *d = v3{c * d->x - s * d->z, d->y, s * d->x + c * d->z};
+    }
+  }
+}
+
+// Inverse mapping of a hit point / normal (innermost op first).
+__device__ __forceinline__ void to_world(const tpt_xform &x, v3 *p, v3 *n) {
+  for (int i = static_cast<int>(x.nops) - 1; i >= 0; --i) {
+    const uint32_t kind = (x.kinds >> (2 * i)) & 3u;
+    if (kind == TPT_XFORM_TRANSLATE) {
+      *p = v3{p->x + x.p[i][0], p->y + x.p[i][1], p->z + x.p[i][2]};
+    } else {
+      const float s = x.p[i][0], c = x.p[i][1];
+      *p = v3{c * p->x + s * p->z, p->y, -s * p->x + c * p->z};
+      *n = v3{c * n->x + s * n->z, n->y, -s * n->x + c * n->z};
+    }
+  }
+}
+
+// LIST leaf: primitives in order, t_max shrinking to each accepted t.
+__device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 o, v3 d, float tmin, float tmax,
+                                       float *t, uint32_t *entry, uint32_t *mat) {
+  const uint32_t first = s.a;
+  const uint32_t count = __float_as_uint(s.f[1]);
+  float cur = tmax;
+  bool found = false;
+  uint32_t cached = kNone;
+  v3 lo = o, ld = d;
+  for (uint32_t e = first; e < first + count; ++e) {
+    const tpt_entry en = S.entries[e];
+    if (en.xform != cached) {
+      lo = o;
+      ld = d;
+      if (en.xform != kNone) to_local(S.xforms[en.xform], &lo, &ld);
+      cached = en.xform;
+    }
+    float th;
+    const bool h = en.b == TPT_SLOT_SPHERE ? sphere_t(en.f[0], en.f[1], en.f[2], en.f[3], lo, ld, tmin, cur, &th)
+                                           : rect_t(en.f, (en.a >> 28) & 3u, lo, ld, tmin, cur, &th);
+    if (h) {
+      cur = th;
+      found = true;
+      *entry = e;
+      *mat = en.a;
+    }
+  }
+  *t = cur;
+  return found;
+}
+
+__device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d, float tmin,
+                                       float tmax, float *t, uint32_t *entry, uint32_t *mat) {
+  if (kind == TPT_SLOT_SPHERE) {
+    *entry = kNone;
+    *mat = s.a;
+    return sphere_t(s.f[0], s.f[1], s.f[2], s.f[3], o, d, tmin, tmax, t);
+  }
+  if (kind == TPT_SLOT_RECT) {
+    *entry = kNone;
+    *mat = s.a;
+    return rect_t(s.f, (s.a >> 28) & 3u, o, d, tmin, tmax, t);
+  }
+  return list_t(S, s, o, d, tmin, tmax, t, entry, mat);
+}
+
+// Exact replay of the bvh_node recursion for one segment: evaluates each node
+// with a fresh result, combining (left, right) as `left.t < right.t ? left :
+// right`.  Stack entries: a pending right child, or a finished left result.
+__device__ __attribute__((noinline)) bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
+                                                        float tmin, float tmax, hit_t *out) {
+  struct frame {
+    uint32_t tag;  // bit31 set: finished-left result; else pending right slot
+    uint32_t slot, entry, mat;
+    float t;
+  };
+  frame stk[64];
+  int sp = 0;
+  bool have = false;
+  hit_t acc{0.0f, kNone, kNone, 0};
+  uint32_t i = start;
+  for (;;) {
+    // Evaluate node i; acc holds its result when this block ends.
+    for (;;) {
+      const slot_t s = load_slot(&S.slots[i]);
+      const uint32_t kind = s.b >> 28;
+      if (kind == TPT_SLOT_BOX) {
+        if (slab(s, o, inv, tmin, tmax)) {
+          const uint32_t right = s.b & TPT_PAYLOAD_MASK;
+          if (right != TPT_NO_RIGHT && sp < 64) stk[sp++] = frame{right, 0, 0, 0, 0.0f};
+          i = i + 1;
+          continue;
+        }
+        have = false;
+      } else {
+        float t;
+        uint32_t entry, mat;
+        have = leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat);
+        acc = hit_t{t, i, entry, mat};
+      }
+      break;
+    }
+    // Unwind: combine finished results, or start a pending right child.
+    bool resumed = false;
+    while (sp > 0) {
+      const frame f = stk[--sp];
+      if (f.tag & 0x80000000u) {
+        const bool lhave = (f.tag & 1u) != 0;
+        if (lhave && have) {
+          if (f.t < acc.t) acc = hit_t{f.t, f.slot, f.entry, f.mat};
+        } else if (lhave) {
+          acc = hit_t{f.t, f.slot, f.entry, f.mat};
+          have = true;
+        }
+      } else {
+        stk[sp++] = frame{0x80000000u | (have ? 1u : 0u), acc.slot, acc.entry, acc.mat, acc.t};
+        have = false;
+        i = f.tag;
+        resumed = true;
+        break;
+      }
+    }
+    if (!resumed) break;
+  }
+  if (have) *out = acc;
+  return have;
+}
+
+// Fast pre-order walk of one segment.  *saw_nan is set when a leaf returned a
+// NaN t (the caller then uses segment_exact).
+__device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
+                                             float tmin, float tmax, hit_t *out, bool *saw_nan, uint32_t *nodes) {
+  bool have = false;
+  hit_t best{0.0f, kNone, kNone, 0};
+  uint32_t i = start;
+  uint32_t visited = 0;
+  while (i < end) {
+    const slot_t s = load_slot(&S.slots[i]);
+    const uint32_t kind = s.b >> 28;
+    ++visited;
+    if (kind == TPT_SLOT_BOX) {
+      i = slab(s, o, inv, tmin, tmax) ? i + 1 : s.a;
+      continue;
+    }
+    float t;
+    uint32_t entry, mat;
+    if (leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
+      if (t != t) *saw_nan = true;
+      if (!have || t <= best.t) {
+        best = hit_t{t, i, entry, mat};
+        have = true;
+      }
+    }
+    ++i;
+  }
+  *nodes += visited;
+  if (have) *out = best;
+  return have;
+}
+
+// World = segments evaluated like a hitable_list (t_max shrinks between them).
+__device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float tmin, float tmax, hit_t *out,
+                                            uint32_t *nodes) {
+  const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  float cur = tmax;
+  bool any = false;
+  for (uint32_t g = 0; g < S.n_segments; ++g) {
+    const uint32_t start = S.segments[2 * g], end = S.segments[2 * g + 1];
+    hit_t h;
+    bool saw_nan = false;
+    if (start >= end) continue;
+    bool got = segment_fast(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+    if (saw_nan) got = segment_exact(S, start, o, d, inv, tmin, cur, &h);
+    if (got) {
+      *out = h;
+      cur = h.t;
+      any = true;
+    }
+  }
+  return any;
+}
+
+// Leaf id reported by tpt_trace_closest.
+__device__ __forceinline__ uint32_t leaf_id(const scene_view &S, const hit_t &h) {
+  const slot_t s = load_slot(&S.slots[h.slot]);
+  if ((s.b >> 28) == TPT_SLOT_LIST) return __float_as_uint(s.f[0]) + (h.entry - s.a);
+  return s.b & TPT_PAYLOAD_MASK;
+}
+
+// Full hit_record of the winner: point, normal (flip applied), u, v.
+__device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, v3 o, v3 d, v3 *p, v3 *n, float *u,
+                                           float *v) {
+  const slot_t s = load_slot(&S.slots[h.slot]);
+  uint32_t kind = s.b >> 28;
+  float f[5] = {s.f[0], s.f[1], s.f[2], s.f[3], s.f[4]};
+  uint32_t word = s.a;
+  v3 lo = o, ld = d;
+  uint32_t xf = kNone;
+  if (kind == TPT_SLOT_LIST) {
+    const tpt_entry en = S.entries[h.entry];
+    for (int k = 0; k < 5; ++k) f[k] = en.f[k];
+    word = en.a;
+    kind = en.b;
+    xf = en.xform;
+    if (xf != kNone) to_local(S.xforms[xf], &lo, &ld);
+  }
+  const float t = h.t;
+  v3 pt{lo.x + t * ld.x, lo.y + t * ld.y, lo.z + t * ld.z};
+  v3 nn;
+  if (kind == TPT_SLOT_SPHERE) {
+    const float r = f[3];
+    const v3 q{(pt.x - f[0]) / r, (pt.y - f[1]) / r, (pt.z - f[2]) / r};
+    // get_uv_map (headers/utils.h:38-43); atan2f/asinf are the device's.
+    *u = static_cast<float>(static_cast<double>(atan2f(q.z, q.x)) / (2 * M_PI)) + 0.5f;
+    *v = static_cast<float>(static_cast<double>(asinf(q.y)) / M_PI) + 0.5f;
+    nn = q;
+  } else {
+    const uint32_t axis = (word >> 28) & 3u;
+    const float x = axis == 0 ? pt.y : pt.x;
+    const float y = axis == 2 ? pt.y : pt.z;
+    *u = (x - f[1]) / (f[2] - f[1]);
+    *v = (y - f[3]) / (f[4] - f[3]);
+    nn = v3{axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+  }
+  if ((word >> 30) & 1u) nn = -nn;
+  if (xf != kNone) to_world(S.xforms[xf], &pt, &nn);
+  *p = pt;
+  *n = nn;
+}
+
+}  // namespace tptd
+
+#endif  // TPT_CSRC_TRAVERSE_H

@@ -87,7 +87,14 @@ class flattener {
   }
 
   void lights(const hitable *l) {
-    if (l) light(l, 1.0f);
+    if (!l) return;
+    out_->light_mode = 0;
+    if (auto *lst = dynamic_cast<const hitable_list *>(l)) {
+      out_->light_mode = 1;
+      for (int i = 0; i < lst->list_size_; ++i)
+        if (dynamic_cast<const hitable_list *>(lst->list_[i])) out_->light_mode = 2;
+    }
+    light(l, 1.0f);
   }
 
  private:
@@ -374,6 +381,7 @@ tpt_flat_scene flat_scene::view() const {
   v.perlin = perlin.empty() ? nullptr : perlin.data();
   v.lights = lights.data();
   v.n_lights = static_cast<uint32_t>(lights.size());
+  v.light_mode = light_mode;
   v.n_leaves = n_leaves;
   v.max_depth = max_depth;
   return v;
