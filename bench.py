@@ -37,7 +37,7 @@ def parse():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--spp", type=int, default=None, help="samples per pixel per step (default: the scene's 2048)")
-    p.add_argument("--cpu-spp", type=int, default=2, help="spp of the bounded CPU baseline sample")
+    p.add_argument("--cpu-spp", type=int, default=256, help="spp of the bounded CPU baseline sample (about 15 s)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--scene", default="cornell555")
     return p.parse_args()
@@ -55,7 +55,7 @@ def cpu_baseline(scene, w, h, spp, depth):
         R = Reference()
         hnd = R.scene(scene)
         t0 = time.perf_counter()
-        R.render(hnd, cam, w, h, spp, depth)
+        R.render(hnd, cam, w, h, spp, depth, decorrelate_rows=False)
         dt = time.perf_counter() - t0
         kind = "reference"
         how = "the reference's own color() and thread-per-row loop (oracle/_ref, g++ -O3)"

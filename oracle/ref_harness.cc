@@ -296,22 +296,33 @@ int ref_bvh_dump(void *sp, float *out, int cap) {
 // std::async task per row, each with a fresh thread_local generator, summing
 // de_nan(color()) per pixel into sum (3·w·h floats, row 0 = bottom).  Camera
 // from params as in ref_camera_fields.  Returns 0, or -1 without a light list.
+// decorrelate_rows: see below (statistical tests only; timing uses 0).
 int ref_render(void *sp, const float *camp, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
-               float *sum) {
+               float *sum, float *sumsq, int decorrelate_rows) {
   auto *s = static_cast<ref_scene_t *>(sp);
   if (!s->lights) return -1;
   camera cam(vec3(camp[0], camp[1], camp[2]), vec3(camp[3], camp[4], camp[5]), vec3(camp[6], camp[7], camp[8]),
              camp[9], camp[10], camp[11], camp[12], camp[13], camp[14]);
   auto row = [&](int j) {
+    // Each row thread starts from the default-seeded generator (the
+    // reference's behaviour).  With decorrelate_rows, row j first consumes
+    // j * 2^20 draws of its own stream, so rows sample independently.
+    if (decorrelate_rows)
+      for (uint64_t k = 0; k < (static_cast<uint64_t>(j) << 20); ++k) (void)drand_r();
     for (uint32_t i = 0; i < w; i++) {
-      vec3 col(0, 0, 0);
+      vec3 col(0, 0, 0), col2(0, 0, 0);
       for (uint32_t k = 0; k < spp; k++) {
         const float u = ((float)i + drand_r()) / (float)w;
         const float v = ((float)j + drand_r()) / (float)h;
         ray r = cam.get_ray(u, v);
-        col += de_nan(color(r, s->render_world, s->lights, 0, static_cast<int>(max_depth)));
+        const vec3 c = de_nan(color(r, s->render_world, s->lights, 0, static_cast<int>(max_depth)));
+        col += c;
+        col2 += c * c;
       }
-      for (int c = 0; c < 3; ++c) sum[(static_cast<size_t>(j) * w + i) * 3 + c] = col[c];
+      for (int c = 0; c < 3; ++c) {
+        sum[(static_cast<size_t>(j) * w + i) * 3 + c] = col[c];
+        if (sumsq) sumsq[(static_cast<size_t>(j) * w + i) * 3 + c] = col2[c];
+      }
     }
   };
   std::vector<std::future<void>> futures;

@@ -152,7 +152,8 @@ class Reference:
                                 C.c_void_p, C.c_void_p, C.c_int]
         L.ref_camera_fields.argtypes = [_f32p, _f32p]
         L.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
-        L.ref_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        L.ref_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p,
+                                 C.c_void_p, C.c_int]
 
     def scene(self, name):
         h = C.c_void_p()
@@ -184,9 +185,13 @@ class Reference:
         n = self.L.ref_bvh_dump(h, out, cap)
         return out[: 6 * min(n, cap)].reshape(-1, 6)
 
-    def render(self, h, cam_params, width, height, spp, max_depth):
+    def render(self, h, cam_params, width, height, spp, max_depth, decorrelate_rows=False, with_sumsq=False):
         s = np.zeros(width * height * 3, np.float32)
-        rc = self.L.ref_render(h, np.asarray(cam_params, np.float32), width, height, spp, max_depth, s)
+        sq = np.zeros(width * height * 3, np.float32)
+        rc = self.L.ref_render(h, np.asarray(cam_params, np.float32), width, height, spp, max_depth, s,
+                               sq.ctypes.data if with_sumsq else None, 1 if decorrelate_rows else 0)
         if rc != 0:
             raise ValueError("reference render needs a light list")
+        if with_sumsq:
+            return s.reshape(height, width, 3), sq.reshape(height, width, 3)
         return s.reshape(height, width, 3)

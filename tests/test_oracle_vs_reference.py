@@ -31,3 +31,18 @@ def test_reference_render_runs(reference):
     cam = [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0]
     img = reference.render(h, cam, 16, 16, 4, 50)
     assert np.isfinite(img).all() and img.sum() > 0
+
+
+def test_oracle_color_matches_reference_color_statistically(oracle, reference):
+    """color() restated vs the reference's color(): per-pixel means within 3 sigma."""
+    w, spp = 32, 64
+    s_o, q_o = oracle.scene("cornell555").render(w, w, spp, seed=4242)
+    h = reference.scene("cornell555")
+    camp = [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0]
+    s_r, q_r = reference.render(h, camp, w, w, spp, 50, decorrelate_rows=True, with_sumsq=True)
+    m_o, m_r = s_o / spp, s_r / spp
+    v = (np.maximum(q_o / spp - m_o ** 2, 0) + np.maximum(q_r / spp - m_r ** 2, 0)) / spp
+    z = (m_o - m_r) / np.sqrt(v + 1e-12)
+    assert np.mean(np.abs(z) > 3) < 0.015
+    se = np.sqrt((q_o.mean() + q_r.mean()) / spp / (w * w * 3 * spp))
+    assert abs(m_o.mean() - m_r.mean()) <= 4 * se

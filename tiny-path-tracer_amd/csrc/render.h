@@ -1,0 +1,466 @@
+// Wavefront renderer kernels (replace color() + the sample loop of the
+// reference: src/utils.cc:58-94, main.cpp:115-142).
+//
+// Work: one render pass covers samples [s0, s1) of every pixel; item w of the
+// pass is (pixel w % n_pix, sample s0 + w / n_pix).  A fixed number of path
+// records is in flight; a finished path writes its de_nan'ed sample to
+// res[w] and claims the next item from a global cursor, so the GPU stays full
+// until the pass runs out of work.  At the end of a pass k_accumulate adds
+// res to the caller's per-pixel sums in sample order, so the result does not
+// depend on scheduling.
+//
+// Queues hold path records by value, as planes of float4 (one 16-byte plane
+// per field group), so every queue access is a coalesced dwordx4:
+//   ray queue (ping-pong), 4 planes: (o, item) (d, depth) (throughput, -)
+//                                    (radiance, -)
+//   material queues (diffuse / specular / terminal), the 4 ray planes plus
+//                                    (t, slot, entry, material)
+// Appends use a 64-lane ballot + popcount prefix and one atomic per wave, on
+// one of 64 sub-queue counters (see NSUB).
+#ifndef TPT_CSRC_RENDER_H
+#define TPT_CSRC_RENDER_H
+
+#include "shade.h"
+
+#pragma clang fp contract(off)
+
+namespace tptd {
+
+enum { Q_DIFFUSE = 0, Q_SPECULAR = 1, Q_TERMINAL = 2 };
+enum { ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_COUNT = 8 };
+
+// Every queue is split into NSUB regions, each with its own counter 128 bytes
+// from the next: block b appends to region b % NSUB.  Wave atomics on one
+// address serialise (about 11 ns each on MI355X); spread over 64 counters they
+// cost about 0.3 ns, so appends stop limiting the bounce loop.  Consumers see
+// the concatenation of the regions through a prefix sum kept in LDS.
+constexpr uint32_t NSUB = 64;
+constexpr uint32_t CSTRIDE = 32;  // words between counters
+// counter word offsets
+__host__ __device__ constexpr uint32_t rc_at(uint32_t parity, uint32_t k) { return (parity * NSUB + k) * CSTRIDE; }
+__host__ __device__ constexpr uint32_t mc_at(uint32_t parity, uint32_t c, uint32_t k) {
+  return (2 * NSUB + (parity * 3 + c) * NSUB + k) * CSTRIDE;
+}
+__host__ __device__ constexpr uint32_t cur_at(uint32_t k) { return (8 * NSUB + k) * CSTRIDE; }
+constexpr uint32_t COUNTER_WORDS = 9 * NSUB * CSTRIDE;
+
+struct render_state {
+  float4 *rq[2][4];    // ray queues by parity, 4 planes; NSUB regions of `region` entries
+  float4 *mq[3][5];    // material queues, 5 planes, same regions
+  uint32_t region;     // entries per region
+  uint32_t *counters;  // COUNTER_WORDS
+  float *res;          // 3 floats per item of the pass
+  unsigned long long *stats;  // ST_COUNT per block
+};
+
+struct render_params {
+  tpt_camera cam;
+  uint32_t width, height, n_pix;
+  uint32_t s0, pass_spp;  // pass: samples [s0, s0 + pass_spp)
+  uint32_t n_items;       // n_pix * pass_spp
+  uint32_t n_first;       // items [0, n_first) seed the first ray queue
+  uint32_t max_depth;
+  uint64_t seed;
+  float background;
+  float *sum, *sumsq;
+};
+
+struct path {
+  v3 o, d, thr, rad;
+  uint32_t item, depth;
+};
+
+__device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
+  return __popcll(mask & ((1ull << __lane_id()) - 1ull));
+}
+
+// One atomic per wave; every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  const uint32_t total = __popcll(mask);
+  uint32_t base = 0;
+  if (total) {
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+  }
+  return base + lane_prefix(mask);
+}
+
+// Items [n_first, n_items) are split into NSUB contiguous shards, one cursor each.
+__device__ __forceinline__ uint32_t shard_lo(const render_params &P, uint32_t k) {
+  return P.n_first + static_cast<uint32_t>((static_cast<uint64_t>(P.n_items - P.n_first) * k) / NSUB);
+}
+
+// Exclusive prefix over n (<= 4 * 64) region counts into pre[0..n]; one wave
+// of the block computes it.  Ends with a block barrier.
+__device__ __forceinline__ void region_prefix(const uint32_t *counters, const uint32_t *offsets, uint32_t n,
+                                              uint32_t *pre) {
+  if (threadIdx.x < 64) {
+    uint32_t v[4] = {0, 0, 0, 0};
+    uint32_t local = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t idx = threadIdx.x * 4 + j;
+      v[j] = idx < n ? counters[offsets[idx]] : 0u;
+      local += v[j];
+    }
+    uint32_t incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t o = __shfl_up(incl, off);
+      if (__lane_id() >= static_cast<uint32_t>(off)) incl += o;
+    }
+    uint32_t run = incl - local;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t idx = threadIdx.x * 4 + j;
+      if (idx <= n) pre[idx] = run;
+      run += v[j];
+    }
+  }
+  __syncthreads();
+}
+
+// Region r with pre[r] <= q < pre[r + 1] (pre has n + 1 entries, q < pre[n]).
+__device__ __forceinline__ uint32_t find_region(const uint32_t *pre, uint32_t n, uint32_t q) {
+  uint32_t lo = 0, hi = n;  // invariant: pre[lo] <= q < pre[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= q) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Lanes with `want` receive the next unclaimed work item (or >= n_items when
+// the pass has none left).  One atomic per wave per shard tried; a wave starts
+// at its own shard and moves on when that one runs dry.  All lanes call.
+__device__ __forceinline__ uint32_t claim_item(const render_params &P, uint32_t *counters, bool want) {
+  unsigned long long need = __ballot(want);
+  uint32_t item = 0xFFFFFFFFu;
+  uint32_t k = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) % NSUB;
+  for (uint32_t attempt = 0; attempt < NSUB && need; ++attempt, k = (k + 1) % NSUB) {
+    const uint32_t hi = shard_lo(P, k + 1);
+    uint32_t *cur = counters + cur_at(k);
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= hi)
+      continue;
+    const uint32_t cnt = __popcll(need);
+    const int leader = __ffsll(static_cast<long long>(need)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(cur, cnt);
+    base = __shfl(base, leader);
+    const uint32_t avail = base < hi ? (hi - base < cnt ? hi - base : cnt) : 0u;
+    const bool mine = (need >> __lane_id()) & 1ull;
+    const uint32_t rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
+    if (mine && rank < avail) item = base + rank;
+    need &= ~__ballot(mine && rank < avail);
+  }
+  return item;
+}
+
+// Adds per-thread counts into this block's stats row (no global atomics).
+__device__ __forceinline__ void block_stats(unsigned long long *stats, const uint32_t *v, int n) {
+  __shared__ unsigned long long acc[ST_COUNT];
+  if (threadIdx.x < ST_COUNT) acc[threadIdx.x] = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    uint32_t x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(&acc[k], static_cast<unsigned long long>(x));
+  }
+  __syncthreads();
+  if (threadIdx.x < static_cast<uint32_t>(n)) stats[blockIdx.x * ST_COUNT + threadIdx.x] += acc[threadIdx.x];
+}
+
+__device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
+  q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
+  q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth));
+  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, 0.0f);
+  q[3][at] = make_float4(p.rad.x, p.rad.y, p.rad.z, 0.0f);
+}
+
+__device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
+  const float4 a = q[0][at], b = q[1][at], c = q[2][at], d = q[3][at];
+  path p;
+  p.o = v3{a.x, a.y, a.z};
+  p.item = __float_as_uint(a.w);
+  p.d = v3{b.x, b.y, b.z};
+  p.depth = __float_as_uint(b.w);
+  p.thr = v3{c.x, c.y, c.z};
+  p.rad = v3{d.x, d.y, d.z};
+  return p;
+}
+
+__device__ __forceinline__ v3 cam_vec(const float *a) { return v3{a[0], a[1], a[2]}; }
+
+// Camera path for work item `item`: jitter (main.cpp:121-122, in double as
+// there), then camera_with_blur::get_ray (src/camera.cc:23-31).  Bounce 0 of
+// the (pixel, sample) stream.
+__device__ __forceinline__ path raygen(const render_params &P, uint32_t item) {
+  const uint32_t px = item % P.n_pix;
+  const uint32_t s = P.s0 + item / P.n_pix;
+  stream R;
+  R.init(px, s, 0, P.seed);
+  const uint32_t i = px % P.width, j = px / P.width;
+  const float u = static_cast<float>((static_cast<double>(static_cast<float>(i)) + static_cast<double>(R.next())) /
+                                     static_cast<double>(static_cast<float>(P.width)));
+  const float v = static_cast<float>((static_cast<double>(static_cast<float>(j)) + static_cast<double>(R.next())) /
+                                     static_cast<double>(static_cast<float>(P.height)));
+  const tpt_camera &C = P.cam;
+  const v3 disk = in_unit_disk(R);
+  const v3 rd = disk * C.lens_radius;
+  const v3 off = cam_vec(C.u) * rd.x + cam_vec(C.v) * rd.y;
+  (void)R.next();  // ray time draw (time0 + xi*(time1 - time0)); static scenes ignore it
+  const v3 org = cam_vec(C.origin);
+  path p;
+  p.o = org + off;
+  p.d = cam_vec(C.lower_left) + u * cam_vec(C.horizontal) + v * cam_vec(C.vertical) - org - off;
+  p.thr = v3{1.0f, 1.0f, 1.0f};
+  p.rad = v3{0.0f, 0.0f, 0.0f};
+  p.item = item;
+  p.depth = 0;
+  return p;
+}
+
+__global__ void k_zero_stats(unsigned long long *stats, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) stats[i] = 0;
+}
+
+// First items of a pass, straight into ray queue 0 (item w -> region w % NSUB,
+// entry w / NSUB; no atomics); resets every counter of the pass.
+__global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t w = gid; w < COUNTER_WORDS / CSTRIDE; w += stride) {
+    const uint32_t set = w / NSUB, k = w % NSUB;
+    uint32_t v = 0;
+    if (set == 0) v = (P.n_first + NSUB - 1 - k) / NSUB;  // ray queue 0 region counts
+    if (set == 8) v = shard_lo(P, k);                       // shard cursors
+    Q.counters[w * CSTRIDE] = v;
+  }
+  for (uint32_t w = gid; w < P.n_first; w += stride)
+    store_ray(Q.rq[0], (w % NSUB) * Q.region + w / NSUB, raygen(P, w));
+}
+
+// SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  The
+// slot/entry arrays come in as separate restrict arguments so that
+// wave-uniform loads from them can use the scalar data path.
+template <bool SWEEP>
+__global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
+                                               const tpt_entry *__restrict__ entries, render_params P,
+                                               render_state Q, uint32_t parity) {
+  S.slots = slots;
+  S.entries = entries;
+  __shared__ uint32_t offs[NSUB];
+  __shared__ uint32_t pre[NSUB + 1];
+  if (threadIdx.x < NSUB) offs[threadIdx.x] = rc_at(parity, threadIdx.x);
+  __syncthreads();
+  region_prefix(Q.counters, offs, NSUB, pre);
+  const uint32_t count = pre[NSUB];
+  // the next ray queue is filled by this iteration's k_shade
+  if (blockIdx.x == 0 && threadIdx.x < NSUB) Q.counters[rc_at(parity ^ 1u, threadIdx.x)] = 0;
+  float4 *const *in = Q.rq[parity];
+  const uint32_t home = blockIdx.x % NSUB;
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    const uint32_t qi = base + threadIdx.x;
+    const bool valid = qi < count;
+    uint32_t cls = 3;
+    path p{};
+    hit_t h{0.0f, kNone, kNone, 0};
+    if (valid) {
+      const uint32_t r = find_region(pre, NSUB, qi);
+      p = load_ray(in, r * Q.region + (qi - pre[r]));
+    }
+    const bool got = trace_world<SWEEP>(S, p.o, p.d, 0.001f, kFltMax, &h, &st[ST_NODES], valid);
+    if (valid) {
+      ++st[ST_SEGMENTS];
+      if (got) {
+        h.mat &= TPT_PAYLOAD_MASK;
+        const uint32_t kind = S.materials[h.mat].kind;
+        if (kind == TPT_MAT_DIFFUSE_LIGHT || p.depth >= P.max_depth) {
+          cls = Q_TERMINAL;
+        } else {
+          cls = kind == TPT_MAT_LAMBERTIAN ? Q_DIFFUSE : Q_SPECULAR;
+        }
+      } else {
+        h.slot = kNone;
+        cls = Q_TERMINAL;
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+      const uint32_t at = home * Q.region + wave_append(Q.counters + mc_at(parity, k, home), cls == k);
+      if (cls == k) {
+        store_ray(Q.mq[k], at, p);
+        Q.mq[k][4][at] = make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
+      }
+    }
+  }
+  block_stats(Q.stats, st, ST_COUNT);
+}
+
+__global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+  // material regions in class-major order: region r = class * NSUB + k
+  __shared__ uint32_t offs[3 * NSUB];
+  __shared__ uint32_t pre[3 * NSUB + 1];
+  if (threadIdx.x < 3 * NSUB) offs[threadIdx.x] = mc_at(parity, threadIdx.x / NSUB, threadIdx.x % NSUB);
+  __syncthreads();
+  region_prefix(Q.counters, offs, 3 * NSUB, pre);
+  const uint32_t total = pre[3 * NSUB];
+  if (blockIdx.x == 0 && threadIdx.x < 3 * NSUB)
+    Q.counters[mc_at(parity ^ 1u, threadIdx.x / NSUB, threadIdx.x % NSUB)] = 0;
+  const uint32_t home = blockIdx.x % NSUB;
+  uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
+  float4 *const *out = Q.rq[parity ^ 1u];
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const uint32_t qi = base + threadIdx.x;
+    const bool valid = qi < total;
+    bool cont = false, finished = false;
+    path p{};
+    if (valid) {
+      const uint32_t r = find_region(pre, 3 * NSUB, qi);
+      const uint32_t cls = r / NSUB;
+      const uint32_t off = (r % NSUB) * Q.region + (qi - pre[r]);
+      p = load_ray(Q.mq[cls], off);
+      const float4 hw = Q.mq[cls][4][off];
+      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      if (h.slot == kNone) {
+        // miss: black (src/utils.cc:85-86) or the sky extension
+        const v3 bg = P.background > 0.0f ? sky(p.d, P.background) : v3{0.0f, 0.0f, 0.0f};
+        p.rad = p.rad + p.thr * bg;
+      } else {
+        v3 pt, n;
+        float u, v;
+        hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+        const tpt_material M = S.materials[h.mat];
+        const uint32_t px = p.item % P.n_pix, smp = P.s0 + p.item / P.n_pix;
+        stream R;
+        R.init(px, smp, p.depth + 1, P.seed);
+        v3 nd{0.0f, 0.0f, 0.0f};
+        if (cls == Q_TERMINAL) {
+          // diffuse_light::emitted is one-sided (src/material.cc:79-86);
+          // every other material emits nothing.
+          v3 e{0.0f, 0.0f, 0.0f};
+          if (M.kind == TPT_MAT_DIFFUSE_LIGHT && dot(n, p.d) < 0.0f) e = tex_value(S, M.tex, u, v, pt);
+          p.rad = p.rad + p.thr * e;
+        } else if (M.kind == TPT_MAT_METAL) {
+          // metal::scatter (src/material.cc:88-98); specular: no emitted term.
+          const v3 refl = reflect(unit(p.d), n);
+          nd = refl + M.p[3] * in_unit_sphere(R);
+          if (dot(nd, n) > 0.0f) {
+            p.thr = p.thr * v3{M.p[0], M.p[1], M.p[2]};
+            cont = true;
+          } else {
+            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
+          }
+        } else if (M.kind == TPT_MAT_DIELECTRIC) {
+          // dielectric::scatter (src/material.cc:19-70).
+          const float ior = M.p[0];
+          const v3 reflected = reflect(p.d, n);
+          v3 outward;
+          float ni, cosine;
+          if (dot(p.d, n) > 0.0f) {
+            outward = -n;
+            ni = ior;
+            cosine = dot(p.d, n) / len(p.d);
+            cosine = sqrtf(1.0f - ior * ior * (1.0f - cosine * cosine));
+          } else {
+            outward = n;
+            ni = 1.0f / ior;
+            cosine = -dot(p.d, n) / len(p.d);
+          }
+          v3 refracted{0.0f, 0.0f, 0.0f};
+          const float prob = refract(p.d, outward, ni, &refracted) ? schlick(cosine, ior) : 1.0f;
+          nd = R.next() < prob ? reflected : refracted;
+          cont = true;
+        } else {
+          // lambertian + mixture_pdf(hitable_pdf(lights), cosine_pdf)
+          // (src/material.cc:3-17, src/utils.cc:73-81); cosine_pdf::value
+          // is cos θ without 1/π, as in the reference.
+          const v3 atten = tex_value(S, M.tex, u, v, pt);
+          const onb uvw(n);
+          float pdf;
+          if (S.n_lights > 0) {
+            if (R.next() < 0.5f) {
+              nd = light_random(S, pt, R);
+            } else {
+              nd = uvw.local(on_hemisphere(R));
+            }
+            const float pl = light_pdf(S, pt, nd);
+            ++st[ST_LIGHT];
+            float pc = dot(unit(nd), uvw.w);
+            pc = pc > 0.0f ? pc : 0.0f;
+            pdf = 0.5f * pl + 0.5f * pc;
+          } else {
+            nd = uvw.local(on_hemisphere(R));
+            const float pc = dot(unit(nd), uvw.w);
+            pdf = pc > 0.0f ? pc : 0.0f;
+          }
+          float spdf = dot(n, unit(nd));
+          spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+          const v3 w = (atten * spdf) / pdf;
+          p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
+          p.thr = p.thr * w;
+          cont = true;
+        }
+        if (cont) {
+          p.o = pt;
+          p.d = nd;
+          p.depth += 1;
+        }
+      }
+      if (!cont) {
+        // de_nan (headers/utils.h:100-109), then the sample goes to res.
+        float c[3] = {p.rad.x, p.rad.y, p.rad.z};
+        bool any_nan = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (c[k] != c[k]) {
+            c[k] = 0.0f;
+            any_nan = true;
+          }
+        st[ST_NAN] += any_nan ? 1u : 0u;
+        ++st[ST_PATHS];
+        float *r = Q.res + 3ull * p.item;
+        r[0] = c[0], r[1] = c[1], r[2] = c[2];
+        finished = true;
+      }
+    }
+    // finished lanes claim the next work item
+    const uint32_t item = claim_item(P, Q.counters, finished);
+    if (finished && item < P.n_items) {
+      p = raygen(P, item);
+      cont = true;
+    }
+    const uint32_t at = home * Q.region + wave_append(next_count, cont);
+    if (cont) store_ray(out, at, p);
+  }
+  block_stats(Q.stats, st, ST_COUNT);
+}
+
+// sum/sumsq += the pass's samples, in sample order per pixel.
+__global__ void __launch_bounds__(256) k_accumulate(render_params P, const float *__restrict__ res) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < P.n_pix; px += gridDim.x * blockDim.x) {
+    float *sum = P.sum + 3ull * px;
+    float a0 = sum[0], a1 = sum[1], a2 = sum[2];
+    float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+    float *sq = P.sumsq ? P.sumsq + 3ull * px : nullptr;
+    if (sq) q0 = sq[0], q1 = sq[1], q2 = sq[2];
+    for (uint32_t s = 0; s < P.pass_spp; ++s) {
+      const float *r = res + 3ull * (static_cast<uint64_t>(s) * P.n_pix + px);
+      const float c0 = r[0], c1 = r[1], c2 = r[2];
+      a0 += c0, a1 += c1, a2 += c2;
+      q0 += c0 * c0, q1 += c1 * c1, q2 += c2 * c2;
+    }
+    sum[0] = a0, sum[1] = a1, sum[2] = a2;
+    if (sq) sq[0] = q0, sq[1] = q1, sq[2] = q2;
+  }
+}
+
+}  // namespace tptd
+
+#endif  // TPT_CSRC_RENDER_H

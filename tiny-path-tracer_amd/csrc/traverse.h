@@ -205,7 +205,7 @@ __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uin
 // Exact replay of the bvh_node recursion for one segment: evaluates each node
 // with a fresh result, combining (left, right) as `left.t < right.t ? left :
 // right`.  Stack entries: a pending right child, or a finished left result.
-__device__ __attribute__((noinline)) bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
+__device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
                                                         float tmin, float tmax, hit_t *out) {
   struct frame {
     uint32_t tag;  // bit31 set: finished-left result; else pending right slot
@@ -296,9 +296,68 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
   return have;
 }
 
+// Smallest value over the active lanes of the wave.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Wave-uniform sweep of one segment, for small scenes: the wave steps through
+// the slots together, always at the smallest slot some lane still has to
+// visit; a lane acts only on the slot equal to its own next pre-order slot.
+// Each lane therefore evaluates exactly the slots of its own pre-order walk,
+// in the same order (same results as segment_fast), while the slot index is
+// wave-uniform: slot data comes through scalar loads and the branch on the
+// slot kind does not diverge.  Lanes with `live` false take no part.
+// Must be called by all lanes of the wave together.
+__device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t start, uint32_t end, bool live, v3 o, v3 d,
+                                              v3 inv, float tmin, float tmax, hit_t *out, bool *saw_nan,
+                                              uint32_t *nodes) {
+  bool have = false;
+  hit_t best{0.0f, kNone, kNone, 0};
+  uint32_t next = live ? start : end;
+  uint32_t visited = 0;
+  uint32_t i = __builtin_amdgcn_readfirstlane(wave_min(next));
+  while (i < end) {
+    const slot_t s = load_slot(&S.slots[i]);
+    const uint32_t kind = s.b >> 28;
+    const bool mine = next == i;
+    if (mine) {
+      ++visited;
+      if (kind == TPT_SLOT_BOX) {
+        next = slab(s, o, inv, tmin, tmax) ? i + 1 : s.a;
+      } else {
+        float t;
+        uint32_t entry, mat;
+        if (leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
+          if (t != t) *saw_nan = true;
+          if (!have || t <= best.t) {
+            best = hit_t{t, i, entry, mat};
+            have = true;
+          }
+        }
+        next = i + 1;
+      }
+    }
+    // Next slot: i + 1 when any lane goes there (the common case), else the
+    // smallest pending skip target.
+    const bool step = __ballot(next == i + 1) != 0ull;
+    i = step ? i + 1 : __builtin_amdgcn_readfirstlane(wave_min(next));
+  }
+  *nodes += visited;
+  if (have) *out = best;
+  return have;
+}
+
 // World = segments evaluated like a hitable_list (t_max shrinks between them).
+// SWEEP selects segment_sweep (all lanes of the wave must call together).
+template <bool SWEEP = false>
 __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float tmin, float tmax, hit_t *out,
-                                            uint32_t *nodes) {
+                                            uint32_t *nodes, bool live = true) {
   const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
   float cur = tmax;
   bool any = false;
@@ -307,8 +366,14 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, flo
     hit_t h;
     bool saw_nan = false;
     if (start >= end) continue;
-    bool got = segment_fast(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
-    if (saw_nan) got = segment_exact(S, start, o, d, inv, tmin, cur, &h);
+    bool got;
+    if (SWEEP) {
+      got = segment_sweep(S, start, end, live, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+    } else {
+      if (!live) continue;
+      got = segment_fast(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+    }
+    if (live && saw_nan) got = segment_exact(S, start, o, d, inv, tmin, cur, &h);
     if (got) {
       *out = h;
       cur = h.t;
