@@ -174,6 +174,7 @@ typedef struct tpt_flat_scene {
                           lists, picked by cumulative weight */
   uint32_t n_leaves; /* number of leaf ids */
   uint32_t max_depth; /* deepest bvh_node nesting */
+  float bounds[6];    /* world bounding_box (min xyz, max xyz); normalises guiding inputs */
 } tpt_flat_scene;
 
 /* Camera fields as computed by camera_with_blur's constructor
@@ -260,7 +261,7 @@ typedef struct tpt_render_desc {
   uint64_t seed;          /* Philox key */
   uint32_t sample_offset; /* first sample index of this pass */
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
-  uint32_t guiding;       /* 0: reference estimator; 1: learned proposal mixed in by one-sample MIS */
+  uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
   uint32_t queue_capacity; /* paths in flight; 0 = automatic */
   uint32_t flags;          /* TPT_RENDER_PROFILE or 0 */
 } tpt_render_desc;
@@ -282,6 +283,54 @@ typedef struct tpt_render_stats {
 
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
                float *d_sumsq, tpt_render_stats *stats, void *stream);
+
+/* ------------------------------------------------------- learned guiding ---
+ * No counterpart in the reference: an online-trained proposal, mixed into the
+ * reference's mixture_pdf(hitable_pdf, cosine_pdf) by one-sample MIS so that
+ * the estimator keeps the reference's expectation.  A 42 -> 3x64 ReLU -> 40
+ * MLP (frequency-encoded hit point, normal, outgoing direction) outputs an
+ * 8-lobe von Mises-Fisher mixture g.  Directions are drawn from
+ * m = a*g + (1-a)*q, q = 0.5*p_light + 0.5*cos/pi (the reference sampler's true
+ * density), and weighted atten*(cos/pi)*q / (p_ref * m), p_ref being the
+ * reference's mixture value 0.5*p_light + 0.5*cos; a = 0 is the reference
+ * estimator exactly.  Forward, backward and Adam run as BF16 MFMA kernels with
+ * FP32 accumulation.
+ *
+ * Parameters: TPT_GUIDE_NPARAM floats (fp32, padded layout: W1 64x48, W2 64x64,
+ * W3 64x64, W4 48x64, b1 64, b2 64, b3 64, b4 48; padding entries stay zero)
+ * followed by the mix-weight logit (a = sigmoid).
+ * Training row (TPT_GUIDE_ROW floats): p_hat[3] (hit point mapped to [0,1]^3
+ * by the scene bounds), n[3], wo[3], omega[3] (unit sampled direction),
+ * c (luminance of the sample's contribution over its proposal density, without
+ * albedo; rows with c <= 0 are ignored), q (reference sampler density at omega),
+ * 2 unused. */
+#define TPT_GUIDE_NPARAM 14576
+#define TPT_GUIDE_ROW 16
+typedef struct tpt_guide tpt_guide;
+
+int tpt_guide_create(uint64_t seed, tpt_guide **out);
+int tpt_guide_destroy(tpt_guide *guide);
+/* host_params: TPT_GUIDE_NPARAM + 1 floats (the last is the mix-weight logit). */
+int tpt_guide_get_params(tpt_guide *guide, float *host_params);
+int tpt_guide_set_params(tpt_guide *guide, const float *host_params);
+/* Raw network outputs for n inputs: d_in n x 9 (p_hat, n, wo), d_out n x 40. */
+int tpt_guide_infer(tpt_guide *guide, const float *d_in, uint64_t n, float *d_out, void *stream);
+/* Gradient of the training loss -(1/n) sum_j c_j log g(omega_j) (and of the
+ * mix-weight loss -(1/n) sum c_j log m(omega_j)) over d_rows (n x
+ * TPT_GUIDE_ROW) into d_grad (TPT_GUIDE_NPARAM + 1 floats); *loss receives the
+ * first loss.  No update. */
+int tpt_guide_gradient(tpt_guide *guide, const float *d_rows, uint64_t n, float *d_grad, float *loss, void *stream);
+/* iters Adam steps (beta 0.9/0.999, eps 1e-8) at learning rate lr, each on the
+ * next n_batch rows of d_rows (n rows, used cyclically).  d_rows == NULL: the
+ * rows recorded by the last tpt_render_guided. */
+int tpt_guide_train(tpt_guide *guide, const float *d_rows, uint64_t n, uint64_t n_batch, uint32_t iters, float lr,
+                    float *loss, void *stream);
+/* tpt_render with the guided estimator.  desc->guiding = N > 0 also records
+ * training rows from every N-th path (2 diffuse vertices each) into the
+ * guide's batch, returned by tpt_guide_batch. */
+int tpt_render_guided(tpt_scene *scene, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc,
+                      float *d_sum, float *d_sumsq, tpt_render_stats *stats, void *stream);
+int tpt_guide_batch(tpt_guide *guide, const float **d_rows, uint64_t *n_rows);
 
 #ifdef __cplusplus
 }

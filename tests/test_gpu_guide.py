@@ -1,0 +1,211 @@
+"""Guiding network kernels against the oracle (oracle/guide_oracle.py), through the C ABI.
+
+- tpt_guide_infer: raw outputs equal the oracle's bf16-emulated forward to FP32
+  accumulation rounding, for ragged n and for n beyond one grid of blocks.
+- tpt_guide_gradient: every parameter block and the mix-weight logit equal the
+  oracle's backward at the same bf16 rounding points.
+- tpt_guide_train: the first Adam step is the textbook update of that gradient,
+  and training on learnable data drives the loss toward the data's entropy.
+- tpt_render_guided: the MIS estimator keeps the plain estimator's expectation
+  (untrained and trained networks), and the recorded training rows are sound.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import guide_oracle as G  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = {"W1": (G.W1, G.W2), "W2": (G.W2, G.W3), "W3": (G.W3, G.W4), "W4": (G.W4, G.B1), "b1": (G.B1, G.B2),
+          "b2": (G.B2, G.B3), "b3": (G.B3, G.B4), "b4": (G.B4, G.NPARAM)}
+
+
+def make_params(seed=5):
+    p = G.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    p[G.W4:G.NPARAM] += rng.normal(0, 0.3, G.NPARAM - G.W4)
+    p[~G.live_mask()] = 0
+    p[G.NPARAM] = 0.3
+    return p.astype(np.float32)
+
+
+def to_dev(a, device):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(device)
+
+
+@pytest.fixture(scope="module")
+def guide(tpt, device):
+    return tpt.Guide(seed=1)
+
+
+@pytest.mark.parametrize("n", [1, 65, 1000, 200000])
+def test_infer_matches_oracle(guide, device, n):
+    p = make_params()
+    guide.set_params(p)
+    assert np.array_equal(guide.get_params(), p)
+    rows = G.random_rows(n, seed=11 + n)
+    out = guide.infer(to_dev(rows[:, :9], device)).cpu().numpy()
+    ref, _ = G.forward(p, rows[:, :9])
+    err = np.abs(out - ref) / (1 + np.abs(ref))
+    # Rows agree to FP32 accumulation rounding, except where a feature or an
+    # activation sits on a bf16 rounding boundary and the device's last bit
+    # (sinf, accumulation order) rounds it the other way: one bf16 ulp of one
+    # operand, which moves the outputs by well under 0.05.
+    off = (err > 1e-4).any(axis=1)
+    print(f"n={n}: max relative error {err.max():.3g}; rows off by a bf16 flip {off.mean():.4f}")
+    assert off.mean() <= 0.01
+    assert err.max() < 0.05
+
+
+def test_infer_empty(guide, tpt):
+    import ctypes as C
+
+    assert tpt.lib().tpt_guide_infer(guide.h, None, 0, None, None) == 0
+    assert tpt.lib().tpt_guide_gradient(guide.h, None, 0, None, C.byref(C.c_float()), None) != 0
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4096, 50000])
+def test_gradient_matches_oracle(guide, device, n):
+    p = make_params()
+    guide.set_params(p)
+    rows = G.random_rows(n, seed=23 + n)
+    g_dev, loss = guide.gradient(to_dev(rows, device))
+    g_dev = g_dev.cpu().numpy().astype(np.float64)
+    g_ref, loss_ref = G.gradient(p, rows, emulate=True)
+    assert abs(loss - loss_ref / n) <= 1e-4 * abs(loss_ref / n)
+    for name, (lo, hi) in BLOCKS.items():
+        a, b = g_dev[lo:hi], g_ref[lo:hi]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"n={n} {name}: relative error {rel:.2e}")
+        assert rel < 1e-2, (name, rel)
+    assert np.all(g_dev[~G.live_mask()] == 0)
+    assert abs(g_dev[G.NPARAM] - g_ref[G.NPARAM]) <= 1e-3 * max(1e-3, abs(g_ref[G.NPARAM]))
+
+
+def test_gradient_ignores_rows_without_contribution(guide, device):
+    p = make_params()
+    guide.set_params(p)
+    rows = G.random_rows(3000, seed=31)
+    keep = rows[:, 12] > 0
+    rows[~keep, 12] = -1.0  # ignored rows, also negative or NaN contributions
+    rows[np.flatnonzero(~keep)[::2], 12] = np.nan
+    g_all, _ = guide.gradient(to_dev(rows, device))
+    g_ref, _ = G.gradient(p, rows)
+    rel = np.linalg.norm(g_all.cpu().numpy() - g_ref) / np.linalg.norm(g_ref)
+    assert rel < 1e-2
+
+
+def test_adam_first_step(guide, device):
+    p = make_params(9)
+    guide.set_params(p)
+    rows = G.random_rows(4096, seed=41)
+    rt = to_dev(rows, device)
+    g_dev, _ = guide.gradient(rt)
+    g_dev = g_dev.cpu().numpy().astype(np.float64)
+    guide.train(rt, n_batch=4096, iters=1, lr=1e-3)
+    q = guide.get_params().astype(np.float64)
+    expect = G.adam_first_step(p, g_dev, 1e-3)
+    err = np.abs(q - expect)
+    print(f"max |param - Adam step| = {err.max():.3g}")
+    assert err.max() < 2e-6
+
+
+def vmf_samples(mu, kappa, rng):
+    n = mu.shape[0]
+    u = rng.uniform(size=n)
+    w = 1 + np.log(u + (1 - u) * np.exp(-2 * kappa)) / kappa
+    phi = rng.uniform(0, 2 * np.pi, n)
+    helper = np.where(np.abs(mu[:, :1]) > 0.9, [[0, 1, 0]], [[1, 0, 0]])
+    t1 = np.cross(mu, helper)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(mu, t1)
+    s = np.sqrt(np.maximum(0, 1 - w * w))[:, None]
+    return s * np.cos(phi)[:, None] * t1 + s * np.sin(phi)[:, None] * t2 + w[:, None] * mu
+
+
+def learnable_rows(n, kappa, seed):
+    """Directions drawn from one vMF lobe whose mean depends on the position."""
+    rng = np.random.default_rng(seed)
+    rows = G.random_rows(n, seed)
+    d = rows[:, 0:3].astype(np.float64) - 0.5 + np.array([0.0, 0.0, 0.05])
+    mu = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rows[:, 9:12] = vmf_samples(mu, kappa, rng)
+    rows[:, 12] = 1.0
+    rows[:, 13] = 1.0 / (4 * np.pi)
+    return rows
+
+
+def test_training_fits_a_position_dependent_lobe(tpt, device):
+    kappa = 20.0
+    rows = learnable_rows(1 << 16, kappa, seed=51)
+    rt = to_dev(rows, device)
+    g = tpt.Guide(seed=3)
+    _, l0 = g.gradient(rt)
+    losses = [g.train(rt, n_batch=1 << 14, iters=100, lr=5e-3) for _ in range(6)]
+    _, l1 = g.gradient(rt)
+    # entropy of a vMF lobe: -log C + 1 - 2κ e^{-2κ} / (1 - e^{-2κ}), C its normaliser
+    e = np.exp(-2 * kappa)
+    best = -np.log(kappa / (2 * np.pi * (1 - e))) + 1 - 2 * kappa * e / (1 - e)
+    print(f"loss {l0:.3f} -> {l1:.3f} (batches {['%.3f' % x for x in losses]}); entropy of the data {best:.3f}")
+    assert l1 < l0 - 1.5
+    assert l1 < best + 0.6
+    # the learned proposal beats the uniform one where it is sampled, so the mix weight rises
+    assert g.get_params()[G.NPARAM] > 0.5
+
+
+def cornell_pair(tpt, w, spp, seed, guide=None, guiding=0):
+    sc = tpt.Scene.named("cornell555")
+    s, q, st = sc.render(width=w, height=w, spp=spp, seed=seed, camera=sc.host.camera, guide=guide, guiding=guiding)
+    return s.cpu().numpy(), q.cpu().numpy(), st, sc
+
+
+def zfrac(s1, q1, s2, q2, spp):
+    m1, m2 = s1 / spp, s2 / spp
+    v1 = np.maximum(q1 / spp - m1 * m1, 0) / spp
+    v2 = np.maximum(q2 / spp - m2 * m2, 0) / spp
+    z = (m1 - m2) / np.sqrt(v1 + v2 + 1e-12)
+    return np.mean(np.abs(z) > 3), z
+
+
+def test_recorded_rows_are_sound(tpt, device):
+    g = tpt.Guide(seed=4)
+    w, spp, stride = 32, 64, 4
+    _, _, st, _ = cornell_pair(tpt, w, spp, 5, guide=g, guiding=stride)
+    rows = g.batch_rows().cpu().numpy()
+    assert rows.shape == ((w * w * spp + stride - 1) // stride * 2, 16)
+    used = rows[:, 12] > 0
+    print(f"{rows.shape[0]} rows, {used.mean():.3f} with a contribution; paths {st['paths']}")
+    assert used.mean() > 0.05
+    r = rows[used]
+    assert np.all(np.isfinite(r))
+    assert np.all((r[:, 0:3] > -1e-3) & (r[:, 0:3] < 1 + 1e-3))
+    for c in (3, 6, 9):
+        assert np.allclose(np.linalg.norm(r[:, c:c + 3], axis=1), 1, atol=1e-4)
+    assert np.all(r[:, 13] > 0)
+
+
+@pytest.mark.parametrize("trained", [False, True])
+def test_guided_render_keeps_the_expectation(tpt, device, trained):
+    g = tpt.Guide(seed=6)
+    w, spp = 32, 4096
+    if trained:
+        for k in range(4):
+            cornell_pair(tpt, w, 64, 100 + k, guide=g, guiding=1)
+            g.train(None, n_batch=1 << 14, iters=50, lr=5e-3)
+    alpha = 1 / (1 + np.exp(-g.get_params()[G.NPARAM]))
+    s_g, q_g, st, _ = cornell_pair(tpt, w, spp, 7, guide=g)
+    s_p, q_p, _, _ = cornell_pair(tpt, w, spp, 8)
+    assert st["paths"] == w * w * spp
+    frac, _ = zfrac(s_g, q_g, s_p, q_p, spp)
+    n = w * w * 3 * spp
+    se = np.sqrt(q_g.mean() / spp / n + q_p.mean() / spp / n)
+    dm = (s_g.mean() - s_p.mean()) / spp
+    print(f"trained={trained} alpha={alpha:.3f}: |z|>3 in {frac:.4f}; image mean diff {dm:.5f} ({dm / se:.2f} SE)")
+    assert frac < 0.015
+    assert abs(dm) <= 4 * se

@@ -20,7 +20,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "HostScene", "Scene", "TptError", "scene_names", "build", "tonemap_ppm"]
+__all__ = ["lib", "HostScene", "Scene", "Guide", "TptError", "scene_names", "build", "tonemap_ppm"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtpt.so")
@@ -47,7 +47,7 @@ class FlatScene(C.Structure):
         ("texels", C.c_void_p), ("n_texels", C.c_uint64),
         ("perlin", C.c_void_p),
         ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("light_mode", C.c_uint32),
-        ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32),
+        ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("bounds", C.c_float * 6),
     ]
 
 
@@ -113,6 +113,16 @@ def lib():
     L.tpt_trace_closest.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp]
     L.tpt_trace_closest_record.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp, i32p, vp]
     L.tpt_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp, C.POINTER(RenderStats), vp]
+    L.tpt_render_guided.argtypes = [vp, vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp,
+                                    C.POINTER(RenderStats), vp]
+    L.tpt_guide_create.argtypes = [u64, C.POINTER(vp)]
+    L.tpt_guide_destroy.argtypes = [vp]
+    L.tpt_guide_get_params.argtypes = [vp, vp]
+    L.tpt_guide_set_params.argtypes = [vp, vp]
+    L.tpt_guide_infer.argtypes = [vp, vp, u64, vp, vp]
+    L.tpt_guide_gradient.argtypes = [vp, vp, u64, vp, C.POINTER(C.c_float), vp]
+    L.tpt_guide_train.argtypes = [vp, vp, u64, u64, u32, f32, C.POINTER(C.c_float), vp]
+    L.tpt_guide_batch.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
     _lib = L
     return L
 
@@ -210,7 +220,7 @@ class Scene:
 
     def render(self, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0, background=None,
                guiding=0, queue_capacity=0, sum_buf=None, sumsq_buf=None, stream=None, camera=None,
-               profile=False):
+               profile=False, guide=None):
         """Renders into (and returns) device tensors sum, sumsq of shape (H, W, 3);
         row 0 is the bottom image row as in the reference."""
         torch = _torch()
@@ -237,9 +247,89 @@ class Scene:
         if (width and width != info.width) or (height and height != info.height):
             if camera is None:
                 raise TptError("a different resolution needs its own camera (aspect ratio)")
-        _check(lib().tpt_render(self.h, C.byref(cam), C.byref(desc), sum_buf.data_ptr(), sumsq_buf.data_ptr(),
-                                C.byref(stats), s), "tpt_render")
+        if guide is not None:
+            _check(lib().tpt_render_guided(self.h, guide.h, C.byref(cam), C.byref(desc), sum_buf.data_ptr(),
+                                           sumsq_buf.data_ptr(), C.byref(stats), s), "tpt_render_guided")
+        else:
+            _check(lib().tpt_render(self.h, C.byref(cam), C.byref(desc), sum_buf.data_ptr(), sumsq_buf.data_ptr(),
+                                    C.byref(stats), s), "tpt_render")
         return sum_buf, sumsq_buf, stats.as_dict()
+
+
+NPARAM = 14576
+ROW = 16
+
+
+class Guide:
+    """The online-trained guiding network (tpt_guide_* in include/tpt.h)."""
+
+    def __init__(self, seed=1):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.tpt_guide_create(seed, C.byref(h)), "tpt_guide_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().tpt_guide_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(NPARAM + 1, np.float32)
+        _check(lib().tpt_guide_get_params(self.h, out.ctypes.data), "tpt_guide_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (NPARAM + 1,)
+        _check(lib().tpt_guide_set_params(self.h, p.ctypes.data), "tpt_guide_set_params")
+
+    def infer(self, inputs):
+        """inputs: (n, 9) float32 device tensor -> (n, 40) raw outputs."""
+        torch = _torch()
+        inputs = inputs.contiguous()
+        out = torch.empty((inputs.shape[0], 40), dtype=torch.float32, device=inputs.device)
+        _check(lib().tpt_guide_infer(self.h, inputs.data_ptr(), inputs.shape[0], out.data_ptr(),
+                                     torch.cuda.current_stream().cuda_stream), "tpt_guide_infer")
+        return out
+
+    def gradient(self, rows):
+        torch = _torch()
+        rows = rows.contiguous()
+        grad = torch.empty(NPARAM + 1, dtype=torch.float32, device=rows.device)
+        loss = C.c_float()
+        _check(lib().tpt_guide_gradient(self.h, rows.data_ptr(), rows.shape[0], grad.data_ptr(), C.byref(loss),
+                                        torch.cuda.current_stream().cuda_stream), "tpt_guide_gradient")
+        return grad, loss.value
+
+    def train(self, rows=None, n_batch=1 << 18, iters=1, lr=1e-3):
+        """Adam steps on `rows` ((n, 16) device tensor) or, if None, on the rows
+        recorded by the last guided render.  Returns the last batch loss."""
+        torch = _torch()
+        loss = C.c_float()
+        ptr, n = (rows.data_ptr(), rows.shape[0]) if rows is not None else (None, 0)
+        _check(lib().tpt_guide_train(self.h, ptr, n, n_batch, iters, lr, C.byref(loss),
+                                     torch.cuda.current_stream().cuda_stream), "tpt_guide_train")
+        return loss.value
+
+    def batch_rows(self):
+        """Copy of the rows recorded by the last guided render, (n, 16) float32 on the device."""
+        torch = _torch()
+        ptr = C.c_void_p()
+        n = C.c_uint64()
+        _check(lib().tpt_guide_batch(self.h, C.byref(ptr), C.byref(n)), "tpt_guide_batch")
+        out = torch.empty((n.value, ROW), dtype=torch.float32, device="cuda")
+        if n.value:
+            torch.cuda.synchronize()
+            hip = C.CDLL("libamdhip64.so")
+            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            rc = hip.hipMemcpy(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3)  # device to device
+            if rc != 0:
+                raise TptError(f"hipMemcpy failed ({rc})")
+        return out
 
 
 def tonemap_ppm(mean_rgb):

@@ -13,6 +13,8 @@
 
 namespace tptd {
 
+constexpr float kPiF = 3.14159265358979323846f;
+
 struct v3 {
   float x, y, z;
 };

@@ -1,0 +1,208 @@
+// Learned path-guiding proposal: network layout, frequency encoding, the
+// 8-lobe von Mises–Fisher mixture, and the BF16 MFMA forward pass.
+//
+// Network (no counterpart in the reference; it is an addition):
+//   x = [sin/cos(2^b π p̂_c) for c in xyz, b in 0..5] (36) ++ n (3) ++ wo (3)   -> 42 inputs
+//   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), h3 = relu(W3 h2 + b3)      64 wide
+//   o  = W4 h3 + b4                                                        40 outputs
+//   lobe i: mean direction normalize(o[5i..5i+2]), κ = exp(clamp(o[5i+3])),
+//   weight softmax(o[5i+4] over i).
+// p̂ is the hit point mapped to [0,1]^3 by the scene box; n the shading normal;
+// wo = -unit(ray direction).  13,672 parameters; 26,880 FLOP per inference.
+//
+// MFMA mapping (v_mfma_f32_16x16x16bf16_1k; lane layout verified on gfx950):
+// layer outputs are computed transposed, Yᵀ[feature][sample] = W·Xᵀ, with
+// A = W (16x16 tiles, one 8-byte LDS read per lane) and B = Xᵀ.  The D tile
+// of one layer (lane l, register r = feature 4·(l/16)+r of sample l%16) is
+// then exactly the B operand of the next layer after bias, ReLU and the
+// bf16 conversion, so activations never leave registers.  Inputs are padded
+// to 48 (3 K-tiles) and outputs to 48 (3 M-tiles).
+#ifndef TPT_CSRC_GUIDE_H
+#define TPT_CSRC_GUIDE_H
+
+#include "dev_common.h"
+
+namespace tptd {
+
+constexpr int G_IN = 42, G_IN_PAD = 48, G_HID = 64, G_OUT = 40, G_OUT_PAD = 48, G_LOBES = 8;
+// bf16 weight tiles (16x16, 512 B) in A-operand order, layer by layer:
+// W1: 4 M-tiles x 3 K-tiles, W2, W3: 4 x 4, W4: 3 x 4.
+constexpr int G_TILES1 = 12, G_TILES2 = 16, G_TILES3 = 16, G_TILES4 = 12;
+constexpr int G_TILES = G_TILES1 + G_TILES2 + G_TILES3 + G_TILES4;  // 56
+constexpr int G_TILE_WORDS = 128;                                     // 512 B = 128 uint32
+constexpr int G_BIAS = G_HID * 3 + G_OUT_PAD;                          // 240 floats
+// fp32 master parameters, padded: W1 64x48, W2 64x64, W3 64x64, W4 48x64, then biases.
+constexpr int G_W1 = 0, G_W2 = G_W1 + 64 * 48, G_W3 = G_W2 + 64 * 64, G_W4 = G_W3 + 64 * 64;
+constexpr int G_B1 = G_W4 + 48 * 64, G_B2 = G_B1 + 64, G_B3 = G_B2 + 64, G_B4 = G_B3 + 64;
+constexpr int G_NPARAM_PAD = G_B4 + 48;  // 14,576 including padding
+constexpr float G_LOGK_MIN = -3.0f, G_LOGK_MAX = 7.0f;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  // round to nearest even (inputs are finite here)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = static_cast<short>(to_bf16(a));
+  v[1] = static_cast<short>(to_bf16(b));
+  v[2] = static_cast<short>(to_bf16(c));
+  v[3] = static_cast<short>(to_bf16(d));
+  return v;
+}
+
+// Feature f (0..47) of an input (p̂, n, wo).
+__device__ __forceinline__ float guide_feature(int f, const float *in9) {
+  if (f < 36) {
+    const int c = f / 12, b = (f % 12) >> 1;
+    const float x = static_cast<float>(1 << b) * kPiF * in9[c];
+    return (f & 1) ? cosf(x) : sinf(x);
+  }
+  if (f < 42) return in9[3 + (f - 36)];
+  return 0.0f;
+}
+
+// Forward pass for 16 samples held by one wave: lane l works on sample l%16
+// and feature group l/16.  `in9` = this lane's sample inputs (p̂, n, wo);
+// `wt` = LDS weight tiles (A-operand order), `bias` = LDS biases.  Returns the
+// output D tiles: out[mt][r] = o[16·mt + 4·(l/16) + r] of sample l%16.  If
+// `acts` is non-null the post-ReLU hidden activations are kept (for backward):
+// acts[layer][mt][r].
+__device__ __forceinline__ void guide_forward16(const float *in9, const uint32_t *wt, const float *bias,
+                                                f32x4 out[3], f32x4 (*acts)[4]) {
+  const int l = __lane_id();
+  const int fg = l >> 4;
+  bf16x4 x[4];
+#pragma unroll
+  for (int kt = 0; kt < 3; ++kt) {
+    const int f = 16 * kt + 4 * fg;
+    x[kt] = pack4(guide_feature(f, in9), guide_feature(f + 1, in9), guide_feature(f + 2, in9),
+                  guide_feature(f + 3, in9));
+  }
+  const bf16x4 *tiles = reinterpret_cast<const bf16x4 *>(wt);
+  int tile = 0;
+  // three hidden layers
+#pragma unroll
+  for (int layer = 0; layer < 3; ++layer) {
+    const int kts = layer == 0 ? 3 : 4;
+    f32x4 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+      if (kt < kts) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tiles[(tile + mt * kts + kt) * 64 + l], x[kt], acc[mt], 0,
+                                                             0, 0);
+      }
+    }
+    tile += 4 * kts;
+    const float *b = bias + 64 * layer;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 h;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = acc[mt][r] + b[16 * mt + 4 * fg + r];
+        h[r] = z > 0.0f ? z : 0.0f;
+      }
+      if (acts) acts[layer][mt] = h;
+      x[mt] = pack4(h[0], h[1], h[2], h[3]);
+    }
+  }
+  // output layer: 3 M-tiles x 4 K-tiles
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt) out[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt)
+      out[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tiles[(tile + mt * 4 + kt) * 64 + l], x[kt], out[mt], 0, 0, 0);
+  const float *b4 = bias + 192;
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[mt][r] += b4[16 * mt + 4 * fg + r];
+}
+
+// ---------------------------------------------------------------- vMF mix ---
+struct vmf_mix {
+  float mu[G_LOBES][3];
+  float kappa[G_LOBES];
+  float w[G_LOBES];
+};
+
+// Decodes 40 raw outputs.
+__device__ __forceinline__ void decode_mix(const float *o, vmf_mix *m) {
+  float mx = -3.0e38f;
+  for (int i = 0; i < G_LOBES; ++i) mx = fmaxf(mx, o[5 * i + 4]);
+  float z = 0.0f;
+  for (int i = 0; i < G_LOBES; ++i) {
+    const float a = o[5 * i], b = o[5 * i + 1], c = o[5 * i + 2];
+    const float n2 = a * a + b * b + c * c;
+    if (n2 > 1e-20f) {
+      const float inv = rsqrtf(n2);
+      m->mu[i][0] = a * inv, m->mu[i][1] = b * inv, m->mu[i][2] = c * inv;
+    } else {
+      m->mu[i][0] = 0.0f, m->mu[i][1] = 0.0f, m->mu[i][2] = 1.0f;
+    }
+    const float lk = fminf(fmaxf(o[5 * i + 3], G_LOGK_MIN), G_LOGK_MAX);
+    m->kappa[i] = __expf(lk);
+    m->w[i] = __expf(o[5 * i + 4] - mx);
+    z += m->w[i];
+  }
+  const float iz = 1.0f / z;
+  for (int i = 0; i < G_LOBES; ++i) m->w[i] *= iz;
+}
+
+// vMF density exp(κ(μ·ω − 1)) · κ / (2π (1 − e^{−2κ})) on the unit sphere.
+__device__ __forceinline__ float vmf_pdf(float kappa, float cosang) {
+  const float norm = kappa / (2.0f * kPiF * (-expm1f(-2.0f * kappa)));
+  return norm * __expf(kappa * (cosang - 1.0f));
+}
+
+__device__ __forceinline__ float mix_pdf(const vmf_mix &m, v3 w) {
+  float g = 0.0f;
+  for (int i = 0; i < G_LOBES; ++i) {
+    const float c = m.mu[i][0] * w.x + m.mu[i][1] * w.y + m.mu[i][2] * w.z;
+    g += m.w[i] * vmf_pdf(m.kappa[i], c);
+  }
+  return g;
+}
+
+// Samples a unit direction: lobe by weight (u0), then the vMF inversion
+// (Wenzel Jakob, "Numerically stable sampling of the von Mises Fisher
+// distribution on S2", 2012): cosθ = 1 + log(u1 + (1 − u1) e^{−2κ}) / κ.
+__device__ __forceinline__ v3 mix_sample(const vmf_mix &m, float u0, float u1, float u2) {
+  int k = G_LOBES - 1;
+  float acc = 0.0f;
+  for (int i = 0; i < G_LOBES; ++i) {
+    acc += m.w[i];
+    if (u0 < acc) {
+      k = i;
+      break;
+    }
+  }
+  const float kap = m.kappa[k];
+  const float ct = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
+  const float c = fminf(fmaxf(ct, -1.0f), 1.0f);
+  const float st = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+  float s, co;
+  __sincosf(2.0f * kPiF * u2, &s, &co);
+  const v3 mu{m.mu[k][0], m.mu[k][1], m.mu[k][2]};
+  const v3 helper = fabsf(mu.x) > 0.9f ? v3{0.0f, 1.0f, 0.0f} : v3{1.0f, 0.0f, 0.0f};
+  const v3 t1 = unit(cross(mu, helper));
+  const v3 t2 = cross(mu, t1);
+  return (st * co) * t1 + (st * s) * t2 + c * mu;
+}
+
+}  // namespace tptd
+
+#endif  // TPT_CSRC_GUIDE_H

@@ -20,6 +20,7 @@
 #ifndef TPT_CSRC_RENDER_H
 #define TPT_CSRC_RENDER_H
 
+#include "guide.h"
 #include "shade.h"
 
 #pragma clang fp contract(off)
@@ -47,6 +48,7 @@ constexpr uint32_t COUNTER_WORDS = 9 * NSUB * CSTRIDE;
 struct render_state {
   float4 *rq[2][4];    // ray queues by parity, 4 planes; NSUB regions of `region` entries
   float4 *mq[3][5];    // material queues, 5 planes, same regions
+  float4 *gq[10];      // guided mode: vMF mixture per diffuse-queue entry (mu 24, kappa 8, w 8)
   uint32_t region;     // entries per region
   uint32_t *counters;  // COUNTER_WORDS
   float *res;          // 3 floats per item of the pass
@@ -63,11 +65,21 @@ struct render_params {
   uint64_t seed;
   float background;
   float *sum, *sumsq;
+  // guided mode
+  const uint32_t *g_tiles;  // bf16 weight tiles
+  const float *g_params;    // fp32 parameters (biases, mix logit)
+  float bounds[6];          // scene box for the position encoding
+  uint32_t rec_stride;      // record training rows from items with item % rec_stride == 0 (0: off)
+  float4 *rec_side;         // per recorded path: 2 vertices x 5 float4
+  float4 *rec_rows;         // training rows: 2 per recorded path, 4 float4 each
 };
+
+constexpr uint32_t REC_VERTS = 2;
 
 struct path {
   v3 o, d, thr, rad;
   uint32_t item, depth;
+  uint32_t nrec;  // training vertices recorded so far (guided mode)
 };
 
 __device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
@@ -175,7 +187,7 @@ __device__ __forceinline__ void block_stats(unsigned long long *stats, const uin
 __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
   q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
   q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth));
-  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, 0.0f);
+  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.nrec));
   q[3][at] = make_float4(p.rad.x, p.rad.y, p.rad.z, 0.0f);
 }
 
@@ -187,6 +199,7 @@ __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
   p.d = v3{b.x, b.y, b.z};
   p.depth = __float_as_uint(b.w);
   p.thr = v3{c.x, c.y, c.z};
+  p.nrec = __float_as_uint(c.w);
   p.rad = v3{d.x, d.y, d.z};
   return p;
 }
@@ -219,6 +232,7 @@ __device__ __forceinline__ path raygen(const render_params &P, uint32_t item) {
   p.rad = v3{0.0f, 0.0f, 0.0f};
   p.item = item;
   p.depth = 0;
+  p.nrec = 0;
   return p;
 }
 
@@ -301,6 +315,110 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
   block_stats(Q.stats, st, ST_COUNT);
 }
 
+// Guided mode: the network's vMF mixture for every diffuse-queue entry, written
+// next to the entry (gq planes, same position).  Each wave evaluates 16
+// entries per MFMA pass (4 lanes per entry); 64 entries per block iteration.
+__global__ void __launch_bounds__(256) k_guide_queue(scene_view S, render_params P, render_state Q, uint32_t parity) {
+  __shared__ uint32_t wt[G_TILES * G_TILE_WORDS];
+  __shared__ float bias[G_BIAS];
+  __shared__ uint32_t offs[NSUB];
+  __shared__ uint32_t pre[NSUB + 1];
+  __shared__ float outs[4][16][48];
+  guide_stage_weights(wt, P.g_tiles, G_TILES * G_TILE_WORDS);
+  for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = P.g_params[G_B1 + i];
+  if (threadIdx.x < NSUB) offs[threadIdx.x] = mc_at(parity, Q_DIFFUSE, threadIdx.x);
+  __syncthreads();
+  region_prefix(Q.counters, offs, NSUB, pre);
+  const uint32_t total = pre[NSUB];
+  const int l = __lane_id(), wave = threadIdx.x / 64, fg = l >> 4, sl = l & 15;
+  for (uint32_t base = blockIdx.x * 64; base < total; base += gridDim.x * 64) {
+    const uint32_t qi = base + wave * 16 + sl;
+    const bool valid = qi < total;
+    float in9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 1};
+    uint32_t pos = 0;
+    if (valid) {
+      const uint32_t r = find_region(pre, NSUB, qi);
+      pos = r * Q.region + (qi - pre[r]);
+      const path pth = load_ray(Q.mq[Q_DIFFUSE], pos);
+      const float4 hw = Q.mq[Q_DIFFUSE][4][pos];
+      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      v3 pt, n;
+      float u, v;
+      hit_record(S, h, pth.o, pth.d, &pt, &n, &u, &v);
+      const v3 wo = -unit(pth.d);
+      const float pv[3] = {pt.x, pt.y, pt.z};
+      for (int k = 0; k < 3; ++k)
+        in9[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
+      in9[3] = n.x, in9[4] = n.y, in9[5] = n.z;
+      in9[6] = wo.x, in9[7] = wo.y, in9[8] = wo.z;
+    }
+    f32x4 o[3];
+    guide_forward16(in9, wt, bias, o, nullptr);
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) outs[wave][sl][16 * mt + 4 * fg + r] = o[mt][r];
+    __syncthreads();
+    if (fg == 0 && valid) {
+      vmf_mix m;
+      decode_mix(outs[wave][sl], &m);
+      float flat[40];  // mu (24), kappa (8), w (8)
+      for (int i = 0; i < G_LOBES; ++i) {
+        flat[3 * i] = m.mu[i][0], flat[3 * i + 1] = m.mu[i][1], flat[3 * i + 2] = m.mu[i][2];
+        flat[24 + i] = m.kappa[i];
+        flat[32 + i] = m.w[i];
+      }
+      for (int k = 0; k < 10; ++k) Q.gq[k][pos] = make_float4(flat[4 * k], flat[4 * k + 1], flat[4 * k + 2], flat[4 * k + 3]);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void load_mix(const render_state &Q, uint32_t pos, vmf_mix *m) {
+  float flat[40];
+  for (int k = 0; k < 10; ++k) {
+    const float4 v = Q.gq[k][pos];
+    flat[4 * k] = v.x, flat[4 * k + 1] = v.y, flat[4 * k + 2] = v.z, flat[4 * k + 3] = v.w;
+  }
+  for (int i = 0; i < G_LOBES; ++i) {
+    m->mu[i][0] = flat[3 * i], m->mu[i][1] = flat[3 * i + 1], m->mu[i][2] = flat[3 * i + 2];
+    m->kappa[i] = flat[24 + i];
+    m->w[i] = flat[32 + i];
+  }
+}
+
+__device__ __forceinline__ float luminance(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Training rows of a finished recorded path: c_k = luminance((R - R_k) / T'_k)
+// with T'_k the throughput times albedo at vertex k (so c is the sample's
+// contribution over its proposal density, without albedo).
+__device__ __forceinline__ void emit_rows(const render_params &P, const path &p) {
+  const uint32_t slot = p.item / P.rec_stride;
+  for (uint32_t k = 0; k < REC_VERTS; ++k) {
+    float4 *row = P.rec_rows + (static_cast<size_t>(slot) * REC_VERTS + k) * 4;
+    if (k < p.nrec) {
+      const float4 *sd = P.rec_side + (static_cast<size_t>(slot) * REC_VERTS + k) * 5;
+      const float4 a = sd[0], b = sd[1], c = sd[2], d = sd[3], e = sd[4];
+      // side: (p̂, n.x) (n.y, n.z, wo.x, wo.y) (wo.z, ω) (q, R) (T', -)
+      const float Rk[3] = {d.y, d.z, d.w};
+      const float Tk[3] = {e.x, e.y, e.z};
+      const float Rf[3] = {p.rad.x, p.rad.y, p.rad.z};
+      float cc[3];
+      for (int ch = 0; ch < 3; ++ch) cc[ch] = Tk[ch] > 0.0f ? (Rf[ch] - Rk[ch]) / Tk[ch] : 0.0f;
+      float cval = 0.2126f * cc[0] + 0.7152f * cc[1] + 0.0722f * cc[2];
+      if (!(cval > 0.0f) || !(cval < 1e30f)) cval = 0.0f;
+      row[0] = make_float4(a.x, a.y, a.z, a.w);
+      row[1] = make_float4(b.x, b.y, b.z, b.w);
+      row[2] = make_float4(c.x, c.y, c.z, c.w);
+      row[3] = make_float4(cval, d.x, 0.0f, 0.0f);
+    } else {
+      row[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      row[0] = row[1] = row[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  }
+}
+
+template <bool GUIDED>
 __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   // material regions in class-major order: region r = class * NSUB + k
   __shared__ uint32_t offs[3 * NSUB];
@@ -383,28 +501,76 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           // is cos θ without 1/π, as in the reference.
           const v3 atten = tex_value(S, M.tex, u, v, pt);
           const onb uvw(n);
-          float pdf;
-          if (S.n_lights > 0) {
-            if (R.next() < 0.5f) {
+          if (!GUIDED) {
+            float pdf;
+            if (S.n_lights > 0) {
+              if (R.next() < 0.5f) {
+                nd = light_random(S, pt, R);
+              } else {
+                nd = uvw.local(on_hemisphere(R));
+              }
+              const float pl = light_pdf(S, pt, nd);
+              ++st[ST_LIGHT];
+              float pc = dot(unit(nd), uvw.w);
+              pc = pc > 0.0f ? pc : 0.0f;
+              pdf = 0.5f * pl + 0.5f * pc;
+            } else {
+              nd = uvw.local(on_hemisphere(R));
+              const float pc = dot(unit(nd), uvw.w);
+              pdf = pc > 0.0f ? pc : 0.0f;
+            }
+            float spdf = dot(n, unit(nd));
+            spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+            const v3 w = (atten * spdf) / pdf;
+            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
+            p.thr = p.thr * w;
+          } else {
+            // One-sample MIS of the guide g with the reference sampler q:
+            // m = a g + (1 - a) q; weight atten (cos/π) q / (p_ref m), where
+            // p_ref is the reference's mixture value (cos without 1/π), so the
+            // expectation is the reference estimator's for any a and g.
+            vmf_mix mix;
+            load_mix(Q, off, &mix);
+            const float alpha = 1.0f / (1.0f + __expf(-P.g_params[G_NPARAM_PAD]));
+            const bool lights = S.n_lights > 0;
+            if (R.next() < alpha) {
+              const float u0 = R.next(), u1 = R.next(), u2 = R.next();
+              nd = mix_sample(mix, u0, u1, u2);
+            } else if (lights && R.next() < 0.5f) {
               nd = light_random(S, pt, R);
             } else {
               nd = uvw.local(on_hemisphere(R));
             }
-            const float pl = light_pdf(S, pt, nd);
-            ++st[ST_LIGHT];
-            float pc = dot(unit(nd), uvw.w);
+            const v3 ud = unit(nd);
+            const float pl = lights ? light_pdf(S, pt, nd) : 0.0f;
+            st[ST_LIGHT] += lights ? 1u : 0u;
+            float pc = dot(ud, uvw.w);
             pc = pc > 0.0f ? pc : 0.0f;
-            pdf = 0.5f * pl + 0.5f * pc;
-          } else {
-            nd = uvw.local(on_hemisphere(R));
-            const float pc = dot(unit(nd), uvw.w);
-            pdf = pc > 0.0f ? pc : 0.0f;
+            const float p_ref = lights ? 0.5f * pl + 0.5f * pc : pc;
+            const float q = lights ? 0.5f * pl + 0.5f * (pc / kPi) : pc / kPi;
+            const float g = mix_pdf(mix, ud);
+            const float m = alpha * g + (1.0f - alpha) * q;
+            float spdf = dot(n, ud);
+            spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+            const v3 w = (atten * (spdf * q)) / (p_ref * m);
+            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
+            if (P.rec_stride && (p.item % P.rec_stride) == 0 && p.nrec < REC_VERTS) {
+              const uint32_t slot = p.item / P.rec_stride;
+              float4 *sd = P.rec_side + (static_cast<size_t>(slot) * REC_VERTS + p.nrec) * 5;
+              const v3 wo = -unit(p.d);
+              float ph[3];
+              const float pv[3] = {pt.x, pt.y, pt.z};
+              for (int k = 0; k < 3; ++k) ph[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
+              const v3 tp = p.thr * atten;
+              sd[0] = make_float4(ph[0], ph[1], ph[2], n.x);
+              sd[1] = make_float4(n.y, n.z, wo.x, wo.y);
+              sd[2] = make_float4(wo.z, ud.x, ud.y, ud.z);
+              sd[3] = make_float4(q, p.rad.x, p.rad.y, p.rad.z);
+              sd[4] = make_float4(tp.x, tp.y, tp.z, 0.0f);
+              p.nrec += 1;
+            }
+            p.thr = p.thr * w;
           }
-          float spdf = dot(n, unit(nd));
-          spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
-          const v3 w = (atten * spdf) / pdf;
-          p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
-          p.thr = p.thr * w;
           cont = true;
         }
         if (cont) {
@@ -427,6 +593,7 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
         ++st[ST_PATHS];
         float *r = Q.res + 3ull * p.item;
         r[0] = c[0], r[1] = c[1], r[2] = c[2];
+        if (GUIDED && P.rec_stride && (p.item % P.rec_stride) == 0) emit_rows(P, p);
         finished = true;
       }
     }

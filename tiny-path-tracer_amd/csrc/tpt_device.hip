@@ -8,7 +8,10 @@
 #include <vector>
 
 #include "../host/errors.h"
+#include "guide_train.h"
 #include "render.h"
+
+#include <random>
 
 using namespace tptd;
 
@@ -96,6 +99,7 @@ struct tpt_scene {
   uint32_t n_slots = 0, n_segments = 0, n_lights = 0, light_mode = 0, n_leaves = 0, max_depth = 0;
   uint32_t n_materials = 0;
   bool sweep = false;  // wave-uniform traversal (small scenes)
+  float bounds[6] = {0, 0, 0, 1, 1, 1};
   // render state, grown on demand
   void *state = nullptr;
   size_t state_bytes = 0;
@@ -161,8 +165,8 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   const size_t per_residue = (stat_blocks + NSUB - 1) / NSUB;
   const size_t region = capacity / NSUB + capacity / stat_blocks + per_residue * 256 + 256;
   const size_t n = region * NSUB;
-  // 2 ray queues x 4 planes + 3 material queues x 5 planes, 16 B per entry
-  const size_t qbytes = n * 16 * (2 * 4 + 3 * 5) + 23 * 256;
+  // 2 ray queues x 4 planes + 3 material queues x 5 planes + 10 guide planes, 16 B per entry
+  const size_t qbytes = n * 16 * (2 * 4 + 3 * 5 + 10) + 40 * 256;  // + rounding of every take()
   const size_t bytes = 256 + COUNTER_WORDS * sizeof(uint32_t) + stat_blocks * ST_COUNT * sizeof(unsigned long long) +
                        qbytes + res_items * 3 * sizeof(float) + 256;
   HIP_TRY(hipMalloc(&s->state, bytes));
@@ -181,10 +185,55 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
     for (int k = 0; k < 4; ++k) Q.rq[p][k] = reinterpret_cast<float4 *>(take(n * 16));
   for (int c = 0; c < 3; ++c)
     for (int k = 0; k < 5; ++k) Q.mq[c][k] = reinterpret_cast<float4 *>(take(n * 16));
+  for (int k = 0; k < 10; ++k) Q.gq[k] = reinterpret_cast<float4 *>(take(n * 16));
   Q.res = reinterpret_cast<float *>(take(res_items * 3 * sizeof(float)));
   s->capacity = capacity;
   s->res_items = res_items;
   s->stat_blocks = stat_blocks;
+  return TPT_OK;
+}
+
+}  // namespace
+
+struct tpt_guide {
+  int device = 0;
+  int n_cu = 256;
+  float *params = nullptr;   // G_NPARAM_PAD + 1 (mix logit)
+  float *adam_m = nullptr, *adam_v = nullptr;
+  uint32_t *tiles = nullptr;   // G_TILES x 128 words
+  uint32_t *ttiles = nullptr;  // G_TT_TILES x 128 words
+  float *partial = nullptr;    // train blocks x G_PARTIAL
+  float *scratch = nullptr;    // gradient / loss staging (G_NPARAM_PAD + 2)
+  int train_blocks = 0;
+  int step = 0;
+  float4 *rec_side = nullptr, *rec_rows = nullptr;
+  uint64_t rec_cap = 0;  // recorded paths the buffers hold
+  uint64_t n_rows = 0;
+};
+
+namespace {
+
+int guide_pack(tpt_guide *g, hipStream_t stream) {
+  hipLaunchKernelGGL(k_guide_pack, dim3((G_B1 + 255) / 256), dim3(256), 0, stream, g->params,
+                     reinterpret_cast<uint16_t *>(g->tiles), reinterpret_cast<uint16_t *>(g->ttiles));
+  HIP_TRY(hipGetLastError());
+  return TPT_OK;
+}
+
+// One fused forward/backward pass over rows [row0, row0 + n) (cyclic in
+// n_total), partials reduced by k_guide_reduce (mode 0: gradient, 1: Adam).
+int guide_step(tpt_guide *g, const float *rows, uint64_t n, uint64_t row0, uint64_t n_total, int mode, float lr,
+               float *grad, float *loss_dev, hipStream_t stream) {
+  const int blocks = static_cast<int>(std::min<uint64_t>((n + 63) / 64, static_cast<uint64_t>(g->train_blocks)));
+  const size_t lds = k_guide_train_lds();
+  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), lds, stream, g->tiles, g->ttiles, g->params, rows, n,
+                     row0, n_total, g->partial);
+  HIP_TRY(hipGetLastError());
+  if (mode == 1) ++g->step;
+  hipLaunchKernelGGL(k_guide_reduce, dim3((G_NPARAM_PAD + 2 + 255) / 256), dim3(256), 0, stream, g->partial, blocks,
+                     g->params, g->adam_m, g->adam_v, grad, loss_dev, mode, lr, g->step);
+  HIP_TRY(hipGetLastError());
+  if (mode == 1) return guide_pack(g, stream);
   return TPT_OK;
 }
 
@@ -285,6 +334,7 @@ int tpt_scene_create(const tpt_flat_scene *f, tpt_scene **out) {
   // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
   // close to each lane's own count only when the whole tree is small.
   s->sweep = f->n_slots <= 64;
+  for (int k = 0; k < 6; ++k) s->bounds[k] = f->bounds[k];
   if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
       hipSuccess) {
     free_scene(s);
@@ -327,13 +377,17 @@ int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min
   return tpt_trace_closest_record(s, d_rays, n, t_min, t_max, d_leaf, d_t, nullptr, nullptr, stream);
 }
 
-int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum, float *d_sumsq,
-               tpt_render_stats *stats, void *stream_ptr) {
+}  // extern "C"
+
+namespace {
+
+int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+                float *d_sumsq, tpt_render_stats *stats, void *stream_ptr) {
   if (!s || !camera || !desc || !d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
   if (desc->width == 0 || desc->height == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: empty image");
   if (desc->flags & ~TPT_RENDER_PROFILE) return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   const bool profile = (desc->flags & TPT_RENDER_PROFILE) != 0;
-  if (desc->guiding != 0) return tpt::fail(TPT_ERR_UNSUPPORTED, "tpt_render: guiding needs the guided build");
+  if (!guide && desc->guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
   const uint64_t n_pix = static_cast<uint64_t>(desc->width) * desc->height;
   if (n_pix > (1ull << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: more than 2^26 pixels");
   if (static_cast<uint64_t>(desc->sample_offset) + desc->spp > 0xFFFFFFFFull)
@@ -365,6 +419,28 @@ int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *de
   P.sumsq = d_sumsq;
   const scene_view S = s->view();
   const render_state &Q = s->Q;
+  if (guide) {
+    P.g_tiles = guide->tiles;
+    P.g_params = guide->params;
+    for (int k = 0; k < 6; ++k) P.bounds[k] = s->bounds[k];
+    P.rec_stride = desc->guiding;
+    if (P.rec_stride) {
+      const uint64_t need = pass_items / P.rec_stride + 1;
+      if (need > guide->rec_cap) {
+        (void)hipFree(guide->rec_side);
+        (void)hipFree(guide->rec_rows);
+        guide->rec_side = guide->rec_rows = nullptr;
+        guide->rec_cap = 0;
+        HIP_TRY(hipMalloc(&guide->rec_side, need * REC_VERTS * 5 * sizeof(float4)));
+        HIP_TRY(hipMalloc(&guide->rec_rows, need * REC_VERTS * 4 * sizeof(float4)));
+        guide->rec_cap = need;
+      }
+      P.rec_side = guide->rec_side;
+      P.rec_rows = guide->rec_rows;
+      guide->n_rows = (std::min<uint64_t>(pass_items, static_cast<uint64_t>(n_pix) * desc->spp) + P.rec_stride - 1) /
+                      P.rec_stride * REC_VERTS;
+    }
+  }
 
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
@@ -407,8 +483,12 @@ int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *de
         hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
       else
         hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
+      if (guide) hipLaunchKernelGGL(k_guide_queue, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
       if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 1], stream));
-      hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
+      if (guide)
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
+      else
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
       if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 2], stream));
       ++batch;
       if ((it % poll) == poll - 1) {
@@ -458,6 +538,158 @@ int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *de
     stats->trace_launches = n_launch;
     stats->shade_launches = n_launch;
   }
+  return TPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum, float *d_sumsq,
+               tpt_render_stats *stats, void *stream) {
+  return render_impl(s, nullptr, camera, desc, d_sum, d_sumsq, stats, stream);
+}
+
+int tpt_render_guided(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc,
+                      float *d_sum, float *d_sumsq, tpt_render_stats *stats, void *stream) {
+  if (!guide) return tpt::fail(TPT_ERR_INVALID, "tpt_render_guided: null guide");
+  return render_impl(s, guide, camera, desc, d_sum, d_sumsq, stats, stream);
+}
+
+int tpt_guide_create(uint64_t seed, tpt_guide **out) {
+  if (!out) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_create: null argument");
+  *out = nullptr;
+  if (tpt_device_count() < 1) return tpt::fail(TPT_ERR_NODEVICE, "tpt_guide_create: no gfx950 device");
+  // He-normal weights (std sqrt(2 / fan_in)), zero biases, mix logit 0 (a = 0.5).
+  std::vector<float> h(G_NPARAM_PAD + 1, 0.0f);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  for (int L = 0; L < 4; ++L) {
+    int off, rows, cols, mts, kts, base;
+    layer_geom(L, &off, &rows, &cols, &mts, &kts, &base);
+    const float fan_in = L == 0 ? static_cast<float>(G_IN) : 64.0f;
+    const float sd = std::sqrt(2.0f / fan_in) * (L == 3 ? 0.1f : 1.0f);
+    for (int o = 0; o < rows; ++o)
+      for (int i = 0; i < cols; ++i) {
+        const int p = off + o * cols + i;
+        const float v = nd(rng) * sd;
+        h[p] = param_live(p) ? v : 0.0f;
+      }
+  }
+  auto *g = new tpt_guide();
+  HIP_TRY(hipGetDevice(&g->device));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, g->device) == hipSuccess) g->n_cu = prop.multiProcessorCount;
+  g->train_blocks = g->n_cu;
+  const size_t np = G_NPARAM_PAD + 1;
+  HIP_TRY(hipMalloc(&g->params, np * sizeof(float)));
+  HIP_TRY(hipMalloc(&g->adam_m, np * sizeof(float)));
+  HIP_TRY(hipMalloc(&g->adam_v, np * sizeof(float)));
+  HIP_TRY(hipMalloc(&g->tiles, G_TILES * G_TILE_WORDS * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&g->ttiles, G_TT_TILES * G_TILE_WORDS * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&g->partial, static_cast<size_t>(g->train_blocks) * G_PARTIAL * sizeof(float)));
+  HIP_TRY(hipMalloc(&g->scratch, (G_NPARAM_PAD + 2) * sizeof(float)));
+  HIP_TRY(hipMemcpy(g->params, h.data(), np * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(g->adam_m, 0, np * sizeof(float)));
+  HIP_TRY(hipMemset(g->adam_v, 0, np * sizeof(float)));
+  HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_guide_train),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(k_guide_train_lds())));
+  const int rc = guide_pack(g, nullptr);
+  if (rc != TPT_OK) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  *out = g;
+  return TPT_OK;
+}
+
+int tpt_guide_destroy(tpt_guide *g) {
+  if (!g) return TPT_OK;
+  (void)hipDeviceSynchronize();
+  (void)hipFree(g->params);
+  (void)hipFree(g->adam_m);
+  (void)hipFree(g->adam_v);
+  (void)hipFree(g->tiles);
+  (void)hipFree(g->ttiles);
+  (void)hipFree(g->partial);
+  (void)hipFree(g->scratch);
+  (void)hipFree(g->rec_side);
+  (void)hipFree(g->rec_rows);
+  delete g;
+  return TPT_OK;
+}
+
+int tpt_guide_get_params(tpt_guide *g, float *host) {
+  if (!g || !host) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_get_params: null argument");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(host, g->params, (G_NPARAM_PAD + 1) * sizeof(float), hipMemcpyDeviceToHost));
+  return TPT_OK;
+}
+
+int tpt_guide_set_params(tpt_guide *g, const float *host) {
+  if (!g || !host) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_set_params: null argument");
+  std::vector<float> h(host, host + G_NPARAM_PAD + 1);
+  for (int p = 0; p < G_NPARAM_PAD; ++p)
+    if (!param_live(p)) h[p] = 0.0f;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(g->params, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(g->adam_m, 0, h.size() * sizeof(float)));
+  HIP_TRY(hipMemset(g->adam_v, 0, h.size() * sizeof(float)));
+  g->step = 0;
+  const int rc = guide_pack(g, nullptr);
+  if (rc != TPT_OK) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  return TPT_OK;
+}
+
+int tpt_guide_infer(tpt_guide *g, const float *d_in, uint64_t n, float *d_out, void *stream) {
+  if (!g || (n && (!d_in || !d_out))) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_infer: null argument");
+  if (n == 0) return TPT_OK;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((n + 63) / 64, static_cast<uint64_t>(g->n_cu) * 8));
+  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), g->tiles, g->params,
+                     d_in, n, d_out);
+  HIP_TRY(hipGetLastError());
+  return TPT_OK;
+}
+
+int tpt_guide_gradient(tpt_guide *g, const float *d_rows, uint64_t n, float *d_grad, float *loss, void *stream) {
+  if (!g || !d_rows || !d_grad || n == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_gradient: bad argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int rc = guide_step(g, d_rows, n, 0, n, 0, 0.0f, d_grad, g->scratch, st);
+  if (rc != TPT_OK) return rc;
+  float l = 0.0f;
+  HIP_TRY(hipMemcpyAsync(&l, g->scratch, sizeof(float), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (loss) *loss = l / static_cast<float>(n);
+  return TPT_OK;
+}
+
+int tpt_guide_train(tpt_guide *g, const float *d_rows, uint64_t n, uint64_t n_batch, uint32_t iters, float lr,
+                    float *loss, void *stream) {
+  if (g && !d_rows) {  // the rows recorded by the last tpt_render_guided
+    d_rows = reinterpret_cast<const float *>(g->rec_rows);
+    n = g->rec_rows ? g->n_rows : 0;
+  }
+  if (!g || !d_rows || n == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_train: no rows");
+  if (n_batch == 0 || n_batch > n) n_batch = n;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint64_t row0 = 0;
+  for (uint32_t it = 0; it < iters; ++it) {
+    const int rc = guide_step(g, d_rows, n_batch, row0, n, 1, lr, g->scratch, g->scratch + G_NPARAM_PAD + 1, st);
+    if (rc != TPT_OK) return rc;
+    row0 = (row0 + n_batch) % n;
+  }
+  if (loss) {
+    float l = 0.0f;
+    HIP_TRY(hipMemcpyAsync(&l, g->scratch + G_NPARAM_PAD + 1, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *loss = l / static_cast<float>(n_batch);
+  }
+  return TPT_OK;
+}
+
+int tpt_guide_batch(tpt_guide *g, const float **d_rows, uint64_t *n_rows) {
+  if (!g || !d_rows || !n_rows) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_batch: null argument");
+  *d_rows = reinterpret_cast<const float *>(g->rec_rows);
+  *n_rows = g->rec_rows ? g->n_rows : 0;
   return TPT_OK;
 }
 

@@ -384,6 +384,7 @@ tpt_flat_scene flat_scene::view() const {
   v.light_mode = light_mode;
   v.n_leaves = n_leaves;
   v.max_depth = max_depth;
+  for (int i = 0; i < 6; ++i) v.bounds[i] = bounds[i];
   return v;
 }
 
@@ -393,6 +394,13 @@ int flatten(const hitable *world, const hitable *lights, flat_scene *out, std::s
     flattener f(out);
     f.world(world);
     f.lights(lights);
+    AABB box;
+    if (world->bounding_box(0, 1, box)) {
+      for (int i = 0; i < 3; ++i) {
+        out->bounds[i] = box.min_.e[i];
+        out->bounds[3 + i] = box.max_.e[i];
+      }
+    }
     return TPT_OK;
   } catch (const flatten_error &e) {
     if (err) *err = e.what();

@@ -36,6 +36,7 @@ struct flat_scene {
   uint32_t light_mode = 0;
   uint32_t n_leaves = 0;
   uint32_t max_depth = 0;
+  float bounds[6] = {0, 0, 0, 1, 1, 1};
 
   tpt_flat_scene view() const;
 };
