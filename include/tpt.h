@@ -264,6 +264,8 @@ typedef struct tpt_render_desc {
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
   uint32_t queue_capacity; /* paths in flight; 0 = automatic */
   uint32_t flags;          /* TPT_RENDER_PROFILE or 0 */
+  uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
+                              the rest with the reference sampler (0 = 2); ignored by tpt_render */
 } tpt_render_desc;
 
 /* flags: time every trace / shade launch with HIP events (fills the *_ms and

@@ -17,7 +17,6 @@ B1 = W4 + 48 * 64
 B2, B3, B4 = B1 + 64, B1 + 128, B1 + 192
 NPARAM = B4 + 48
 LOGK_MIN, LOGK_MAX = -3.0, 7.0
-PI_F = np.float32(np.pi)
 
 
 def bf16(x):
@@ -44,15 +43,22 @@ def live_mask():
 
 def features(in9):
     """(n, 9) -> (n, 48): sin/cos(2^b pi p_c) for c in xyz, b in 0..5, then n, wo, zeros.
-    The angle is formed in float32 as on the device."""
+    Band 0 is sin/cos(pi p) rounded to float32; band b+1 follows from band b by
+    the float32 doublings sin 2x = (2 sin x) cos x, cos 2x = 1 - (2 sin x) sin x,
+    evaluated in that order as the device does."""
     in9 = np.asarray(in9, np.float32)
     n = in9.shape[0]
     f = np.zeros((n, 48), np.float64)
+    two, one = np.float32(2), np.float32(1)
     for c in range(3):
+        p = in9[:, c].astype(np.float64)
+        s = np.sin(np.pi * p).astype(np.float32)
+        co = np.cos(np.pi * p).astype(np.float32)
         for b in range(6):
-            x = (np.float32(1 << b) * PI_F * in9[:, c]).astype(np.float32).astype(np.float64)
-            f[:, 12 * c + 2 * b] = np.sin(x)
-            f[:, 12 * c + 2 * b + 1] = np.cos(x)
+            f[:, 12 * c + 2 * b] = s
+            f[:, 12 * c + 2 * b + 1] = co
+            s2 = two * s
+            s, co = s2 * co, one - s2 * s
     f[:, 36:42] = in9[:, 3:9]
     return f
 

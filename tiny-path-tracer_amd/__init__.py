@@ -74,7 +74,8 @@ class SceneInfo(C.Structure):
 class RenderDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
                 ("seed", C.c_uint64), ("sample_offset", C.c_uint32), ("background", C.c_float),
-                ("guiding", C.c_uint32), ("queue_capacity", C.c_uint32), ("flags", C.c_uint32)]
+                ("guiding", C.c_uint32), ("queue_capacity", C.c_uint32), ("flags", C.c_uint32),
+                ("guide_vertices", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -220,7 +221,7 @@ class Scene:
 
     def render(self, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0, background=None,
                guiding=0, queue_capacity=0, sum_buf=None, sumsq_buf=None, stream=None, camera=None,
-               profile=False, guide=None):
+               profile=False, guide=None, guide_vertices=0):
         """Renders into (and returns) device tensors sum, sumsq of shape (H, W, 3);
         row 0 is the bottom image row as in the reference."""
         torch = _torch()
@@ -236,6 +237,7 @@ class Scene:
         desc.guiding = guiding
         desc.queue_capacity = queue_capacity
         desc.flags = 1 if profile else 0
+        desc.guide_vertices = guide_vertices
         shape = (desc.height, desc.width, 3)
         if sum_buf is None:
             sum_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")

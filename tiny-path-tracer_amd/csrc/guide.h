@@ -57,15 +57,139 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
   return v;
 }
 
-// Feature f (0..47) of an input (p̂, n, wo).
+// Frequency band b of coordinate p: sin and cos of 2^b π p, from sinpi/cospi
+// of p by b angle doublings, sin 2x = (2 sin x) cos x, cos 2x = 1 - (2 sin x) sin x
+// (each a single rounding in this order).
+__device__ __forceinline__ void enc_trig(float p, int b, float *sn, float *cs) {
+  float s = sinpif(p), c = cospif(p);
+  for (int k = 0; k < b; ++k) {
+    const float s2 = 2.0f * s;
+    const float ns = s2 * c;
+    c = 1.0f - s2 * s;
+    s = ns;
+  }
+  *sn = s;
+  *cs = c;
+}
+
+// Feature f (0..47) of an input (p̂, n, wo): f = 12 c + 2 b + e for the bands
+// (e = 0 sin, 1 cos of 2^b π p̂_c), then n (36..38), wo (39..41), zeros.
 __device__ __forceinline__ float guide_feature(int f, const float *in9) {
   if (f < 36) {
-    const int c = f / 12, b = (f % 12) >> 1;
-    const float x = static_cast<float>(1 << b) * kPiF * in9[c];
-    return (f & 1) ? cosf(x) : sinf(x);
+    float sn, cs;
+    enc_trig(in9[f / 12], (f % 12) >> 1, &sn, &cs);
+    return (f & 1) ? cs : sn;
   }
   if (f < 42) return in9[3 + (f - 36)];
   return 0.0f;
+}
+
+// ------------------------------------------- inference layout (16x16x32) ---
+// Inference uses v_mfma_f32_16x16x32_bf16 (gfx950): lane l holds 8 bf16 of
+// A row l%16 and of B column l%16, for K slots (l/16, q), q = 0..7; D is laid
+// out as in the 16x16x16 form (lane l, register r = row 4(l/16) + r, column
+// l%16).  The K order of a layer is free (it is a sum), so each layer's
+// weight columns are stored permuted such that
+//  - hidden layers: K step s, lane group g, register q reads neuron
+//    16(2s + q/4) + 4g + q%4, which is register q%4 of D tile 2s + q/4 of the
+//    same lane: the previous layer's D tiles are the next B operand as they are;
+//  - first layer: lane group g computes its own 16 features (xslot_feature),
+//    one coordinate's 12 bands from one sinpi/cospi, so nothing is exchanged.
+// Tiles: W1 4 M x 2 K, W2 and W3 4 x 2, W4 3 x 2: 30 tiles of 1 KB.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int GX_TILES = 30, GX_TILE_WORDS = 256;
+constexpr int GX_BASE[4] = {0, 8, 16, 24};
+
+// Feature index held by first-layer K slot t (0..15) of lane group g, or -1.
+__host__ __device__ inline int xslot_feature(int g, int t) {
+  if (g < 3 && t < 12) return 12 * g + t;
+  if (g == 0 && t >= 12) return 36 + (t - 12);  // n, wo.x
+  if (g == 1 && t < 14) return 40 + (t - 12);   // wo.y, wo.z
+  return -1;
+}
+// Inverse: first-layer K slot (g, t) of feature f < G_IN.
+__host__ __device__ inline void feature_xslot(int f, int *g, int *t) {
+  if (f < 36) *g = f / 12, *t = f % 12;
+  else if (f < 40) *g = 0, *t = 12 + (f - 36);
+  else *g = 1, *t = 12 + (f - 40);
+}
+
+__device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
+  bf16x8 v;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    v[r] = __builtin_bit_cast(__bf16, to_bf16(a[r]));
+    v[4 + r] = __builtin_bit_cast(__bf16, to_bf16(b[r]));
+  }
+  return v;
+}
+
+// First-layer B operands of lane group g for one sample: p = its coordinate g
+// (g < 3), nwo = (n, wo).
+__device__ __forceinline__ void guide_xfeatures(int g, float p, const float *nwo, bf16x8 xb[2]) {
+  float v[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) v[t] = 0.0f;
+  if (g < 3) {
+    float s = sinpif(p), c = cospif(p);
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      v[2 * b] = s;
+      v[2 * b + 1] = c;
+      const float s2 = 2.0f * s;
+      const float ns = s2 * c;
+      c = 1.0f - s2 * s;
+      s = ns;
+    }
+    if (g == 0) v[12] = nwo[0], v[13] = nwo[1], v[14] = nwo[2], v[15] = nwo[3];
+    if (g == 1) v[12] = nwo[4], v[13] = nwo[5];
+  }
+  xb[0] = pack8(f32x4{v[0], v[1], v[2], v[3]}, f32x4{v[4], v[5], v[6], v[7]});
+  xb[1] = pack8(f32x4{v[8], v[9], v[10], v[11]}, f32x4{v[12], v[13], v[14], v[15]});
+}
+
+// Forward pass of 16 samples (one per lane column l%16) in the 16x16x32
+// layout: xb = this lane's first-layer B operands, xt = LDS inference tiles,
+// bias = LDS biases.  out[mt][r] = output 16 mt + 4 (l/16) + r.
+__device__ __forceinline__ void guide_forward_x32(const bf16x8 xb_in[2], const uint32_t *xt, const float *bias,
+                                                  f32x4 out[3]) {
+  const int l = __lane_id();
+  const int fg = l >> 4;
+  const bf16x8 *tiles = reinterpret_cast<const bf16x8 *>(xt);
+  bf16x8 xb[2] = {xb_in[0], xb_in[1]};
+#pragma unroll
+  for (int layer = 0; layer < 3; ++layer) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tiles[(GX_BASE[layer] + 2 * mt + s) * 64 + l], xb[s],
+                                                          acc[mt], 0, 0, 0);
+    }
+    const float *b = bias + 64 * layer;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = acc[mt][r] + b[16 * mt + 4 * fg + r];
+        acc[mt][r] = z > 0.0f ? z : 0.0f;
+      }
+    xb[0] = pack8(acc[0], acc[1]);
+    xb[1] = pack8(acc[2], acc[3]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt) {
+    out[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      out[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tiles[(GX_BASE[3] + 2 * mt + s) * 64 + l], xb[s], out[mt], 0,
+                                                        0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[mt][r] += bias[192 + 16 * mt + 4 * fg + r];
+  }
 }
 
 // Forward pass for 16 samples held by one wave: lane l works on sample l%16
@@ -116,6 +240,9 @@ __device__ __forceinline__ void guide_forward16(const float *in9, const uint32_t
       if (acts) acts[layer][mt] = h;
       x[mt] = pack4(h[0], h[1], h[2], h[3]);
     }
+    // keep the next layer's weight loads from being hoisted above this one
+    // (they would hold up to 32 more registers)
+    __builtin_amdgcn_sched_barrier(0);
   }
   // output layer: 3 M-tiles x 4 K-tiles
 #pragma unroll
@@ -137,13 +264,16 @@ struct vmf_mix {
   float mu[G_LOBES][3];
   float kappa[G_LOBES];
   float w[G_LOBES];
+  float wn[G_LOBES];  // w_i times the lobe's normaliser (mix_norms), for mix_pdf
 };
 
 // Decodes 40 raw outputs.
 __device__ __forceinline__ void decode_mix(const float *o, vmf_mix *m) {
   float mx = -3.0e38f;
+#pragma unroll
   for (int i = 0; i < G_LOBES; ++i) mx = fmaxf(mx, o[5 * i + 4]);
   float z = 0.0f;
+#pragma unroll
   for (int i = 0; i < G_LOBES; ++i) {
     const float a = o[5 * i], b = o[5 * i + 1], c = o[5 * i + 2];
     const float n2 = a * a + b * b + c * c;
@@ -159,7 +289,44 @@ __device__ __forceinline__ void decode_mix(const float *o, vmf_mix *m) {
     z += m->w[i];
   }
   const float iz = 1.0f / z;
+#pragma unroll
   for (int i = 0; i < G_LOBES; ++i) m->w[i] *= iz;
+}
+
+// decode_mix from 40 bf16 outputs (an LDS row), reading each output where it
+// is used; false if an output is not finite.
+__device__ __forceinline__ bool decode_mix_bf16(const uint16_t *o, vmf_mix *m) {
+  float lg[G_LOBES];
+  float mx = -3.0e38f;
+  bool finite = true;
+#pragma unroll
+  for (int i = 0; i < G_LOBES; ++i) {
+    lg[i] = from_bf16(o[5 * i + 4]);
+    finite = finite && fabsf(lg[i]) < 3.0e38f;
+    mx = fmaxf(mx, lg[i]);
+  }
+  float z = 0.0f;
+#pragma unroll
+  for (int i = 0; i < G_LOBES; ++i) {
+    const float a = from_bf16(o[5 * i]), b = from_bf16(o[5 * i + 1]), c = from_bf16(o[5 * i + 2]);
+    const float lk0 = from_bf16(o[5 * i + 3]);
+    finite = finite && fabsf(a) < 3.0e38f && fabsf(b) < 3.0e38f && fabsf(c) < 3.0e38f && fabsf(lk0) < 3.0e38f;
+    const float n2 = a * a + b * b + c * c;
+    if (n2 > 1e-20f) {
+      const float inv = rsqrtf(n2);
+      m->mu[i][0] = a * inv, m->mu[i][1] = b * inv, m->mu[i][2] = c * inv;
+    } else {
+      m->mu[i][0] = 0.0f, m->mu[i][1] = 0.0f, m->mu[i][2] = 1.0f;
+    }
+    const float lk = fminf(fmaxf(lk0, G_LOGK_MIN), G_LOGK_MAX);
+    m->kappa[i] = __expf(lk);
+    m->w[i] = __expf(lg[i] - mx);
+    z += m->w[i];
+  }
+  const float iz = 1.0f / z;
+#pragma unroll
+  for (int i = 0; i < G_LOBES; ++i) m->w[i] *= iz;
+  return finite;
 }
 
 // vMF density exp(κ(μ·ω − 1)) · κ / (2π (1 − e^{−2κ})) on the unit sphere.
@@ -168,11 +335,19 @@ __device__ __forceinline__ float vmf_pdf(float kappa, float cosang) {
   return norm * __expf(kappa * (cosang - 1.0f));
 }
 
+// Fills wn: w_i κ_i / (2π (1 − e^{−2κ_i})).
+__device__ __forceinline__ void mix_norms(vmf_mix *m) {
+#pragma unroll
+  for (int i = 0; i < G_LOBES; ++i) m->wn[i] = m->w[i] * (m->kappa[i] / (2.0f * kPiF * (-expm1f(-2.0f * m->kappa[i]))));
+}
+
+// Mixture density at unit direction w (needs mix_norms).
 __device__ __forceinline__ float mix_pdf(const vmf_mix &m, v3 w) {
   float g = 0.0f;
+#pragma unroll
   for (int i = 0; i < G_LOBES; ++i) {
     const float c = m.mu[i][0] * w.x + m.mu[i][1] * w.y + m.mu[i][2] * w.z;
-    g += m.w[i] * vmf_pdf(m.kappa[i], c);
+    g += m.wn[i] * __expf(m.kappa[i] * (c - 1.0f));
   }
   return g;
 }
@@ -181,22 +356,27 @@ __device__ __forceinline__ float mix_pdf(const vmf_mix &m, v3 w) {
 // (Wenzel Jakob, "Numerically stable sampling of the von Mises Fisher
 // distribution on S2", 2012): cosθ = 1 + log(u1 + (1 − u1) e^{−2κ}) / κ.
 __device__ __forceinline__ v3 mix_sample(const vmf_mix &m, float u0, float u1, float u2) {
-  int k = G_LOBES - 1;
-  float acc = 0.0f;
-  for (int i = 0; i < G_LOBES; ++i) {
+  // lobe by the running sum of weights (the last lobe takes any rounding
+  // remainder); selected without dynamic indexing so the mixture stays in
+  // registers
+  float acc = 0.0f, kap = m.kappa[G_LOBES - 1];
+  v3 mu{m.mu[G_LOBES - 1][0], m.mu[G_LOBES - 1][1], m.mu[G_LOBES - 1][2]};
+  bool found = false;
+#pragma unroll
+  for (int i = 0; i < G_LOBES - 1; ++i) {
     acc += m.w[i];
-    if (u0 < acc) {
-      k = i;
-      break;
-    }
+    const bool take = !found && u0 < acc;
+    kap = take ? m.kappa[i] : kap;
+    mu.x = take ? m.mu[i][0] : mu.x;
+    mu.y = take ? m.mu[i][1] : mu.y;
+    mu.z = take ? m.mu[i][2] : mu.z;
+    found = found || take;
   }
-  const float kap = m.kappa[k];
   const float ct = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
   const float c = fminf(fmaxf(ct, -1.0f), 1.0f);
   const float st = sqrtf(fmaxf(0.0f, 1.0f - c * c));
   float s, co;
   __sincosf(2.0f * kPiF * u2, &s, &co);
-  const v3 mu{m.mu[k][0], m.mu[k][1], m.mu[k][2]};
   const v3 helper = fabsf(mu.x) > 0.9f ? v3{0.0f, 1.0f, 0.0f} : v3{1.0f, 0.0f, 0.0f};
   const v3 t1 = unit(cross(mu, helper));
   const v3 t2 = cross(mu, t1);

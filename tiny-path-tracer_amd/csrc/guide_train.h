@@ -43,9 +43,12 @@ __host__ __device__ inline bool param_live(int p) {
   return true;
 }
 
-// bf16 tiles from fp32 master: forward tiles (A = W) and transposed tiles
-// (A = Wᵀ, for layers 4, 3, 2), one thread per (layer, out, in).
-__global__ void k_guide_pack(const float *__restrict__ w, uint16_t *__restrict__ tiles, uint16_t *__restrict__ ttiles) {
+// bf16 tiles from fp32 master: training forward tiles (A = W), transposed
+// tiles (A = Wᵀ, for layers 4, 3, 2) and the inference tiles (16x16x32
+// layout, permuted K order), one thread per (layer, out, in).  Inference-tile
+// slots that hold no weight are never written (they stay zero).
+__global__ void k_guide_pack(const float *__restrict__ w, uint16_t *__restrict__ tiles, uint16_t *__restrict__ ttiles,
+                             uint16_t *__restrict__ xtiles) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G_B1) return;
   int L = t < G_W2 ? 0 : (t < G_W3 ? 1 : (t < G_W4 ? 2 : 3));
@@ -63,19 +66,32 @@ __global__ void k_guide_pack(const float *__restrict__ w, uint16_t *__restrict__
     const int tt = tb + (i / 16) * tkts + (o / 16);
     ttiles[(tt * 64 + (i % 16) + 16 * ((o % 16) / 4)) * 4 + (o % 4)] = v;
   }
+  // inference: K slot (s, g, q) of the input column i
+  int g, s, q;
+  if (L == 0) {
+    if (!param_live(t)) return;
+    int slot;
+    feature_xslot(i, &g, &slot);
+    s = slot / 8, q = slot % 8;
+  } else {
+    const int mt = i / 16;
+    g = (i % 16) / 4, s = mt / 2, q = 4 * (mt % 2) + (i % 4);
+  }
+  const int xt = GX_BASE[L] + (o / 16) * 2 + s;
+  xtiles[(xt * 64 + (o % 16) + 16 * g) * 8 + q] = v;
 }
 
 __device__ __forceinline__ void guide_stage_weights(uint32_t *dst, const uint32_t *src, int words) {
   for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
 }
 
-// Raw outputs for inputs in9 (n x 9) -> out (n x 40).  256 threads, 64 inputs
-// per block-iteration (16 per wave).
-__global__ void __launch_bounds__(256) k_guide_infer(const uint32_t *__restrict__ tiles, const float *__restrict__ params,
+// Raw outputs for inputs in9 (n x 9) -> out (n x 40), on the inference tiles.
+// 256 threads, 64 inputs per block-iteration (16 per wave).
+__global__ void __launch_bounds__(256) k_guide_infer(const uint32_t *__restrict__ xtiles, const float *__restrict__ params,
                                                      const float *__restrict__ in, uint64_t n, float *__restrict__ out) {
-  __shared__ uint32_t wt[G_TILES * G_TILE_WORDS];
+  __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
   __shared__ float bias[G_BIAS];
-  guide_stage_weights(wt, tiles, G_TILES * G_TILE_WORDS);
+  guide_stage_weights(wt, xtiles, GX_TILES * GX_TILE_WORDS);
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = params[G_B1 + i];
   __syncthreads();
   const int l = __lane_id(), wave = threadIdx.x / 64, fg = l >> 4;
@@ -83,8 +99,10 @@ __global__ void __launch_bounds__(256) k_guide_infer(const uint32_t *__restrict_
     const uint64_t s = base + wave * 16 + (l & 15);
     float in9[9];
     for (int k = 0; k < 9; ++k) in9[k] = s < n ? in[s * 9 + k] : 0.0f;
+    bf16x8 xb[2];
+    guide_xfeatures(fg, fg == 0 ? in9[0] : (fg == 1 ? in9[1] : in9[2]), in9 + 3, xb);
     f32x4 o[3];
-    guide_forward16(in9, wt, bias, o, nullptr);
+    guide_forward_x32(xb, wt, bias, o);
     if (s < n) {
 #pragma unroll
       for (int mt = 0; mt < 3; ++mt)

@@ -27,7 +27,9 @@
 
 namespace tptd {
 
-enum { Q_DIFFUSE = 0, Q_SPECULAR = 1, Q_TERMINAL = 2 };
+// Material queues: lambertian hits (the guided ones in guided mode), every other
+// scattering hit, and hits that end the path (emitters, depth limit).
+enum { Q_DIFFUSE = 0, Q_OTHER = 1, Q_TERMINAL = 2 };
 enum { ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_COUNT = 8 };
 
 // Every queue is split into NSUB regions, each with its own counter 128 bytes
@@ -48,7 +50,6 @@ constexpr uint32_t COUNTER_WORDS = 9 * NSUB * CSTRIDE;
 struct render_state {
   float4 *rq[2][4];    // ray queues by parity, 4 planes; NSUB regions of `region` entries
   float4 *mq[3][5];    // material queues, 5 planes, same regions
-  float4 *gq[10];      // guided mode: vMF mixture per diffuse-queue entry (mu 24, kappa 8, w 8)
   uint32_t region;     // entries per region
   uint32_t *counters;  // COUNTER_WORDS
   float *res;          // 3 floats per item of the pass
@@ -66,9 +67,10 @@ struct render_params {
   float background;
   float *sum, *sumsq;
   // guided mode
-  const uint32_t *g_tiles;  // bf16 weight tiles
+  const uint32_t *g_tiles;  // bf16 inference weight tiles (16x16x32 layout)
   const float *g_params;    // fp32 parameters (biases, mix logit)
   float bounds[6];          // scene box for the position encoding
+  uint32_t guide_verts;     // diffuse vertices per path that use the guide (~0u: every one, plain mode)
   uint32_t rec_stride;      // record training rows from items with item % rec_stride == 0 (0: off)
   float4 *rec_side;         // per recorded path: 2 vertices x 5 float4
   float4 *rec_rows;         // training rows: 2 per recorded path, 4 float4 each
@@ -79,7 +81,7 @@ constexpr uint32_t REC_VERTS = 2;
 struct path {
   v3 o, d, thr, rad;
   uint32_t item, depth;
-  uint32_t nrec;  // training vertices recorded so far (guided mode)
+  uint32_t ndiff;  // diffuse (lambertian) vertices so far
 };
 
 __device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
@@ -187,7 +189,7 @@ __device__ __forceinline__ void block_stats(unsigned long long *stats, const uin
 __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
   q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
   q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth));
-  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.nrec));
+  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff));
   q[3][at] = make_float4(p.rad.x, p.rad.y, p.rad.z, 0.0f);
 }
 
@@ -199,7 +201,7 @@ __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
   p.d = v3{b.x, b.y, b.z};
   p.depth = __float_as_uint(b.w);
   p.thr = v3{c.x, c.y, c.z};
-  p.nrec = __float_as_uint(c.w);
+  p.ndiff = __float_as_uint(c.w);
   p.rad = v3{d.x, d.y, d.z};
   return p;
 }
@@ -232,7 +234,7 @@ __device__ __forceinline__ path raygen(const render_params &P, uint32_t item) {
   p.rad = v3{0.0f, 0.0f, 0.0f};
   p.item = item;
   p.depth = 0;
-  p.nrec = 0;
+  p.ndiff = 0;
   return p;
 }
 
@@ -296,7 +298,10 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
         if (kind == TPT_MAT_DIFFUSE_LIGHT || p.depth >= P.max_depth) {
           cls = Q_TERMINAL;
         } else {
-          cls = kind == TPT_MAT_LAMBERTIAN ? Q_DIFFUSE : Q_SPECULAR;
+          // In guided mode only the first guide_verts diffuse vertices of a
+          // path go to the diffuse queue (where the network runs); later
+          // ones are shaded with the plain sampler from the other queue.
+          cls = kind == TPT_MAT_LAMBERTIAN && p.ndiff < P.guide_verts ? Q_DIFFUSE : Q_OTHER;
         }
       } else {
         h.slot = kNone;
@@ -315,78 +320,6 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
   block_stats(Q.stats, st, ST_COUNT);
 }
 
-// Guided mode: the network's vMF mixture for every diffuse-queue entry, written
-// next to the entry (gq planes, same position).  Each wave evaluates 16
-// entries per MFMA pass (4 lanes per entry); 64 entries per block iteration.
-__global__ void __launch_bounds__(256) k_guide_queue(scene_view S, render_params P, render_state Q, uint32_t parity) {
-  __shared__ uint32_t wt[G_TILES * G_TILE_WORDS];
-  __shared__ float bias[G_BIAS];
-  __shared__ uint32_t offs[NSUB];
-  __shared__ uint32_t pre[NSUB + 1];
-  __shared__ float outs[4][16][48];
-  guide_stage_weights(wt, P.g_tiles, G_TILES * G_TILE_WORDS);
-  for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = P.g_params[G_B1 + i];
-  if (threadIdx.x < NSUB) offs[threadIdx.x] = mc_at(parity, Q_DIFFUSE, threadIdx.x);
-  __syncthreads();
-  region_prefix(Q.counters, offs, NSUB, pre);
-  const uint32_t total = pre[NSUB];
-  const int l = __lane_id(), wave = threadIdx.x / 64, fg = l >> 4, sl = l & 15;
-  for (uint32_t base = blockIdx.x * 64; base < total; base += gridDim.x * 64) {
-    const uint32_t qi = base + wave * 16 + sl;
-    const bool valid = qi < total;
-    float in9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 1};
-    uint32_t pos = 0;
-    if (valid) {
-      const uint32_t r = find_region(pre, NSUB, qi);
-      pos = r * Q.region + (qi - pre[r]);
-      const path pth = load_ray(Q.mq[Q_DIFFUSE], pos);
-      const float4 hw = Q.mq[Q_DIFFUSE][4][pos];
-      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
-      v3 pt, n;
-      float u, v;
-      hit_record(S, h, pth.o, pth.d, &pt, &n, &u, &v);
-      const v3 wo = -unit(pth.d);
-      const float pv[3] = {pt.x, pt.y, pt.z};
-      for (int k = 0; k < 3; ++k)
-        in9[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
-      in9[3] = n.x, in9[4] = n.y, in9[5] = n.z;
-      in9[6] = wo.x, in9[7] = wo.y, in9[8] = wo.z;
-    }
-    f32x4 o[3];
-    guide_forward16(in9, wt, bias, o, nullptr);
-#pragma unroll
-    for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) outs[wave][sl][16 * mt + 4 * fg + r] = o[mt][r];
-    __syncthreads();
-    if (fg == 0 && valid) {
-      vmf_mix m;
-      decode_mix(outs[wave][sl], &m);
-      float flat[40];  // mu (24), kappa (8), w (8)
-      for (int i = 0; i < G_LOBES; ++i) {
-        flat[3 * i] = m.mu[i][0], flat[3 * i + 1] = m.mu[i][1], flat[3 * i + 2] = m.mu[i][2];
-        flat[24 + i] = m.kappa[i];
-        flat[32 + i] = m.w[i];
-      }
-      for (int k = 0; k < 10; ++k) Q.gq[k][pos] = make_float4(flat[4 * k], flat[4 * k + 1], flat[4 * k + 2], flat[4 * k + 3]);
-    }
-    __syncthreads();
-  }
-}
-
-__device__ __forceinline__ void load_mix(const render_state &Q, uint32_t pos, vmf_mix *m) {
-  float flat[40];
-  for (int k = 0; k < 10; ++k) {
-    const float4 v = Q.gq[k][pos];
-    flat[4 * k] = v.x, flat[4 * k + 1] = v.y, flat[4 * k + 2] = v.z, flat[4 * k + 3] = v.w;
-  }
-  for (int i = 0; i < G_LOBES; ++i) {
-    m->mu[i][0] = flat[3 * i], m->mu[i][1] = flat[3 * i + 1], m->mu[i][2] = flat[3 * i + 2];
-    m->kappa[i] = flat[24 + i];
-    m->w[i] = flat[32 + i];
-  }
-}
-
 __device__ __forceinline__ float luminance(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // Training rows of a finished recorded path: c_k = luminance((R - R_k) / T'_k)
@@ -396,7 +329,7 @@ __device__ __forceinline__ void emit_rows(const render_params &P, const path &p)
   const uint32_t slot = p.item / P.rec_stride;
   for (uint32_t k = 0; k < REC_VERTS; ++k) {
     float4 *row = P.rec_rows + (static_cast<size_t>(slot) * REC_VERTS + k) * 4;
-    if (k < p.nrec) {
+    if (k < min(p.ndiff, min(REC_VERTS, P.guide_verts))) {
       const float4 *sd = P.rec_side + (static_cast<size_t>(slot) * REC_VERTS + k) * 5;
       const float4 a = sd[0], b = sd[1], c = sd[2], d = sd[3], e = sd[4];
       // side: (p̂, n.x) (n.y, n.z, wo.x, wo.y) (wo.z, ω) (q, R) (T', -)
@@ -418,15 +351,53 @@ __device__ __forceinline__ void emit_rows(const render_params &P, const path &p)
   }
 }
 
+// A finished path: de_nan (headers/utils.h:100-109), then the sample goes to
+// res; in guided mode a recorded path also writes its training rows.
+template <bool GUIDED>
+__device__ __forceinline__ void finish_sample(const render_params &P, const render_state &Q, const path &p,
+                                              uint32_t *st) {
+  float c[3] = {p.rad.x, p.rad.y, p.rad.z};
+  bool any_nan = false;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (c[k] != c[k]) {
+      c[k] = 0.0f;
+      any_nan = true;
+    }
+  st[ST_NAN] += any_nan ? 1u : 0u;
+  ++st[ST_PATHS];
+  float *r = Q.res + 3ull * p.item;
+  r[0] = c[0], r[1] = c[1], r[2] = c[2];
+  if (GUIDED && P.rec_stride && (p.item % P.rec_stride) == 0) emit_rows(P, p);
+}
+
+// Finished lanes claim the next work item; every continuing path goes to the
+// next ray queue.  All lanes call.
+__device__ __forceinline__ void claim_and_append(const render_params &P, const render_state &Q, uint32_t *next_count,
+                                                 float4 *const *out, uint32_t home, path &p, bool cont,
+                                                 bool finished) {
+  const uint32_t item = claim_item(P, Q.counters, finished);
+  if (finished && item < P.n_items) {
+    p = raygen(P, item);
+    cont = true;
+  }
+  const uint32_t at = home * Q.region + wave_append(next_count, cont);
+  if (cont) store_ray(out, at, p);
+}
+
+// Shading of the material queues.  Plain mode: all three classes.  Guided
+// mode: the other and terminal classes; the diffuse class (the vertices that
+// use the guide) is shaded by k_guide_shade.
 template <bool GUIDED>
 __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
-  // material regions in class-major order: region r = class * NSUB + k
-  __shared__ uint32_t offs[3 * NSUB];
-  __shared__ uint32_t pre[3 * NSUB + 1];
-  if (threadIdx.x < 3 * NSUB) offs[threadIdx.x] = mc_at(parity, threadIdx.x / NSUB, threadIdx.x % NSUB);
+  // material regions in class-major order: region r = (class - C0) * NSUB + k
+  constexpr uint32_t C0 = GUIDED ? 1u : 0u, NR = (3u - C0) * NSUB;
+  __shared__ uint32_t offs[NR];
+  __shared__ uint32_t pre[NR + 1];
+  if (threadIdx.x < NR) offs[threadIdx.x] = mc_at(parity, C0 + threadIdx.x / NSUB, threadIdx.x % NSUB);
   __syncthreads();
-  region_prefix(Q.counters, offs, 3 * NSUB, pre);
-  const uint32_t total = pre[3 * NSUB];
+  region_prefix(Q.counters, offs, NR, pre);
+  const uint32_t total = pre[NR];
   if (blockIdx.x == 0 && threadIdx.x < 3 * NSUB)
     Q.counters[mc_at(parity ^ 1u, threadIdx.x / NSUB, threadIdx.x % NSUB)] = 0;
   const uint32_t home = blockIdx.x % NSUB;
@@ -440,8 +411,8 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
     bool cont = false, finished = false;
     path p{};
     if (valid) {
-      const uint32_t r = find_region(pre, 3 * NSUB, qi);
-      const uint32_t cls = r / NSUB;
+      const uint32_t r = find_region(pre, NR, qi);
+      const uint32_t cls = C0 + r / NSUB;
       const uint32_t off = (r % NSUB) * Q.region + (qi - pre[r]);
       p = load_ray(Q.mq[cls], off);
       const float4 hw = Q.mq[cls][4][off];
@@ -501,77 +472,30 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           // is cos θ without 1/π, as in the reference.
           const v3 atten = tex_value(S, M.tex, u, v, pt);
           const onb uvw(n);
-          if (!GUIDED) {
-            float pdf;
-            if (S.n_lights > 0) {
-              if (R.next() < 0.5f) {
-                nd = light_random(S, pt, R);
-              } else {
-                nd = uvw.local(on_hemisphere(R));
-              }
-              const float pl = light_pdf(S, pt, nd);
-              ++st[ST_LIGHT];
-              float pc = dot(unit(nd), uvw.w);
-              pc = pc > 0.0f ? pc : 0.0f;
-              pdf = 0.5f * pl + 0.5f * pc;
-            } else {
-              nd = uvw.local(on_hemisphere(R));
-              const float pc = dot(unit(nd), uvw.w);
-              pdf = pc > 0.0f ? pc : 0.0f;
-            }
-            float spdf = dot(n, unit(nd));
-            spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
-            const v3 w = (atten * spdf) / pdf;
-            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
-            p.thr = p.thr * w;
-          } else {
-            // One-sample MIS of the guide g with the reference sampler q:
-            // m = a g + (1 - a) q; weight atten (cos/π) q / (p_ref m), where
-            // p_ref is the reference's mixture value (cos without 1/π), so the
-            // expectation is the reference estimator's for any a and g.
-            vmf_mix mix;
-            load_mix(Q, off, &mix);
-            const float alpha = 1.0f / (1.0f + __expf(-P.g_params[G_NPARAM_PAD]));
-            const bool lights = S.n_lights > 0;
-            if (R.next() < alpha) {
-              const float u0 = R.next(), u1 = R.next(), u2 = R.next();
-              nd = mix_sample(mix, u0, u1, u2);
-            } else if (lights && R.next() < 0.5f) {
+          float pdf;
+          if (S.n_lights > 0) {
+            if (R.next() < 0.5f) {
               nd = light_random(S, pt, R);
             } else {
               nd = uvw.local(on_hemisphere(R));
             }
-            const v3 ud = unit(nd);
-            const float pl = lights ? light_pdf(S, pt, nd) : 0.0f;
-            st[ST_LIGHT] += lights ? 1u : 0u;
-            float pc = dot(ud, uvw.w);
+            const float pl = light_pdf(S, pt, nd);
+            ++st[ST_LIGHT];
+            float pc = dot(unit(nd), uvw.w);
             pc = pc > 0.0f ? pc : 0.0f;
-            const float p_ref = lights ? 0.5f * pl + 0.5f * pc : pc;
-            const float q = lights ? 0.5f * pl + 0.5f * (pc / kPi) : pc / kPi;
-            const float g = mix_pdf(mix, ud);
-            const float m = alpha * g + (1.0f - alpha) * q;
-            float spdf = dot(n, ud);
-            spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
-            const v3 w = (atten * (spdf * q)) / (p_ref * m);
-            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
-            if (P.rec_stride && (p.item % P.rec_stride) == 0 && p.nrec < REC_VERTS) {
-              const uint32_t slot = p.item / P.rec_stride;
-              float4 *sd = P.rec_side + (static_cast<size_t>(slot) * REC_VERTS + p.nrec) * 5;
-              const v3 wo = -unit(p.d);
-              float ph[3];
-              const float pv[3] = {pt.x, pt.y, pt.z};
-              for (int k = 0; k < 3; ++k) ph[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
-              const v3 tp = p.thr * atten;
-              sd[0] = make_float4(ph[0], ph[1], ph[2], n.x);
-              sd[1] = make_float4(n.y, n.z, wo.x, wo.y);
-              sd[2] = make_float4(wo.z, ud.x, ud.y, ud.z);
-              sd[3] = make_float4(q, p.rad.x, p.rad.y, p.rad.z);
-              sd[4] = make_float4(tp.x, tp.y, tp.z, 0.0f);
-              p.nrec += 1;
-            }
-            p.thr = p.thr * w;
+            pdf = 0.5f * pl + 0.5f * pc;
+          } else {
+            nd = uvw.local(on_hemisphere(R));
+            const float pc = dot(unit(nd), uvw.w);
+            pdf = pc > 0.0f ? pc : 0.0f;
           }
+          float spdf = dot(n, unit(nd));
+          spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+          const v3 w = (atten * spdf) / pdf;
+          p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
+          p.thr = p.thr * w;
           cont = true;
+          p.ndiff += 1;
         }
         if (cont) {
           p.o = pt;
@@ -580,31 +504,166 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
         }
       }
       if (!cont) {
-        // de_nan (headers/utils.h:100-109), then the sample goes to res.
-        float c[3] = {p.rad.x, p.rad.y, p.rad.z};
-        bool any_nan = false;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          if (c[k] != c[k]) {
-            c[k] = 0.0f;
-            any_nan = true;
-          }
-        st[ST_NAN] += any_nan ? 1u : 0u;
-        ++st[ST_PATHS];
-        float *r = Q.res + 3ull * p.item;
-        r[0] = c[0], r[1] = c[1], r[2] = c[2];
-        if (GUIDED && P.rec_stride && (p.item % P.rec_stride) == 0) emit_rows(P, p);
+        finish_sample<GUIDED>(P, Q, p, st);
         finished = true;
       }
     }
-    // finished lanes claim the next work item
-    const uint32_t item = claim_item(P, Q.counters, finished);
-    if (finished && item < P.n_items) {
-      p = raygen(P, item);
-      cont = true;
+    claim_and_append(P, Q, next_count, out, home, p, cont, finished);
+  }
+  block_stats(Q.stats, st, ST_COUNT);
+}
+
+// Guided mode, diffuse queue: network inference fused with the guided
+// shading.  Each lane gathers one entry (path, hit record, encoding inputs);
+// the wave then runs four MFMA passes of 16 entries, lane l taking the inputs
+// of entry 16j + l%16 by shuffle.  A pass leaves outputs 16mt + 4(l/16) .. +3
+// of that entry in lane l's D tiles; they go to the wave's rows of an LDS
+// table (bf16) from which every lane reads its own entry's 40 outputs.
+//
+// One-sample MIS of the guide g with the reference sampler q (the
+// reference's mixture_pdf(hitable_pdf, cosine_pdf), src/material.cc:3-17,
+// src/utils.cc:73-81): directions come from m = a g + (1 - a) q and are
+// weighted atten (cos/π) q / (p_ref m), p_ref being the reference's mixture
+// value (cosine_pdf::value is cos θ without 1/π), so the expectation is the
+// reference estimator's for any a and g.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+  __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
+  __shared__ float bias[G_BIAS];
+  __shared__ uint32_t offs[NSUB];
+  __shared__ uint32_t pre[NSUB + 1];
+  __shared__ uint2 outs[256 * 10];  // entry e: outputs 4k..4k+3 as bf16 in outs[10 e + k]
+  guide_stage_weights(wt, P.g_tiles, GX_TILES * GX_TILE_WORDS);
+  for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = P.g_params[G_B1 + i];
+  if (threadIdx.x < NSUB) offs[threadIdx.x] = mc_at(parity, Q_DIFFUSE, threadIdx.x);
+  __syncthreads();
+  region_prefix(Q.counters, offs, NSUB, pre);
+  const uint32_t total = pre[NSUB];
+  const float alpha_net = 1.0f / (1.0f + __expf(-P.g_params[G_NPARAM_PAD]));
+  const int l = __lane_id(), fg = l >> 4, sl = l & 15;
+  const uint32_t wave_base = (threadIdx.x / 64) * 64;
+  const uint32_t home = blockIdx.x % NSUB;
+  uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
+  float4 *const *out = Q.rq[parity ^ 1u];
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bool lights = S.n_lights > 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t qi = base + threadIdx.x;
+    const bool valid = qi < total;
+    // Phase 1: encoding inputs of this lane's entry (only its queue position
+    // stays live across the network passes; phase 2 reloads the entry).
+    uint32_t off = 0;
+    float in9[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 1.0f};
+    if (valid) {
+      const uint32_t r = find_region(pre, NSUB, qi);
+      off = r * Q.region + (qi - pre[r]);
+      const path p = load_ray(Q.mq[Q_DIFFUSE], off);
+      const float4 hw = Q.mq[Q_DIFFUSE][4][off];
+      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      v3 pt, n;
+      float u, v;
+      hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+      const v3 wo = -unit(p.d);
+      const float pv[3] = {pt.x, pt.y, pt.z};
+      for (int k = 0; k < 3; ++k) in9[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
+      in9[3] = n.x, in9[4] = n.y, in9[5] = n.z;
+      in9[6] = wo.x, in9[7] = wo.y, in9[8] = wo.z;
+      if (P.rec_stride && (p.item % P.rec_stride) == 0 && p.ndiff < REC_VERTS) {
+        // training-row side data: encoding inputs now, the rest after sampling
+        float4 *sd = P.rec_side + (static_cast<size_t>(p.item / P.rec_stride) * REC_VERTS + p.ndiff) * 5;
+        sd[0] = make_float4(in9[0], in9[1], in9[2], in9[3]);
+        sd[1] = make_float4(in9[4], in9[5], in9[6], in9[7]);
+      }
     }
-    const uint32_t at = home * Q.region + wave_append(next_count, cont);
-    if (cont) store_ray(out, at, p);
+#pragma unroll 1
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (base + wave_base + 16 * j >= total) break;  // wave-uniform
+      const int src = static_cast<int>(16 * j) + sl;
+      float x9[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) x9[k] = __shfl(in9[k], src);
+      bf16x8 xb[2];
+      guide_xfeatures(fg, fg == 0 ? x9[0] : (fg == 1 ? x9[1] : x9[2]), x9 + 3, xb);
+      f32x4 o[3];
+      guide_forward_x32(xb, wt, bias, o);
+      uint2 *row = outs + (wave_base + 16 * j + sl) * 10;
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt) {
+        const int k = 4 * mt + fg;
+        if (k < 10) {
+          const bf16x4 b = pack4(o[mt][0], o[mt][1], o[mt][2], o[mt][3]);
+          row[k] = make_uint2(static_cast<uint32_t>(static_cast<uint16_t>(b[0])) |
+                                  (static_cast<uint32_t>(static_cast<uint16_t>(b[1])) << 16),
+                              static_cast<uint32_t>(static_cast<uint16_t>(b[2])) |
+                                  (static_cast<uint32_t>(static_cast<uint16_t>(b[3])) << 16));
+        }
+      }
+    }
+    __syncthreads();
+    // Phase 2: guided shading of this lane's entry.
+    bool cont = false, finished = false;
+    path p{};
+    if (valid) {
+      p = load_ray(Q.mq[Q_DIFFUSE], off);
+      const float4 hw = Q.mq[Q_DIFFUSE][4][off];
+      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      v3 pt, n;
+      float u, v;
+      hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+      vmf_mix mix;
+      const bool finite = decode_mix_bf16(reinterpret_cast<const uint16_t *>(outs + threadIdx.x * 10), &mix);
+      mix_norms(&mix);
+      // a non-finite network output falls back to the reference sampler at this vertex
+      const float alpha = finite ? alpha_net : 0.0f;
+      const tpt_material M = S.materials[h.mat];
+      const uint32_t px = p.item % P.n_pix, smp = P.s0 + p.item / P.n_pix;
+      stream R;
+      R.init(px, smp, p.depth + 1, P.seed);
+      const v3 atten = tex_value(S, M.tex, u, v, pt);
+      const onb uvw(n);
+      v3 nd;
+      if (R.next() < alpha) {
+        const float u0 = R.next(), u1 = R.next(), u2 = R.next();
+        nd = mix_sample(mix, u0, u1, u2);
+      } else if (lights && R.next() < 0.5f) {
+        nd = light_random(S, pt, R);
+      } else {
+        nd = uvw.local(on_hemisphere(R));
+      }
+      const v3 ud = unit(nd);
+      // the mixture density first, so the mixture is dead during light_pdf
+      const float g = alpha > 0.0f ? mix_pdf(mix, ud) : 0.0f;
+      const float pl = lights ? light_pdf(S, pt, nd) : 0.0f;
+      st[ST_LIGHT] += lights ? 1u : 0u;
+      float pc = dot(ud, uvw.w);
+      pc = pc > 0.0f ? pc : 0.0f;
+      const float p_ref = lights ? 0.5f * pl + 0.5f * pc : pc;
+      const float q = lights ? 0.5f * pl + 0.5f * (pc / kPi) : pc / kPi;
+      const float m = alpha > 0.0f ? alpha * g + (1.0f - alpha) * q : q;
+      float spdf = dot(n, ud);
+      spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+      const v3 w = (atten * (spdf * q)) / (p_ref * m);
+      if (P.rec_stride && (p.item % P.rec_stride) == 0 && p.ndiff < REC_VERTS) {
+        float4 *sd = P.rec_side + (static_cast<size_t>(p.item / P.rec_stride) * REC_VERTS + p.ndiff) * 5;
+        const v3 tp = p.thr * atten;
+        sd[2] = make_float4(-unit(p.d).z, ud.x, ud.y, ud.z);
+        sd[3] = make_float4(q, p.rad.x, p.rad.y, p.rad.z);
+        sd[4] = make_float4(tp.x, tp.y, tp.z, 0.0f);
+      }
+      p.thr = p.thr * w;
+      p.ndiff += 1;
+      // Below the surface and off the light the integrand is zero (the
+      // reference sampler never goes there): the path ends with what it has.
+      cont = spdf > 0.0f && q > 0.0f;
+      if (cont) {
+        p.o = pt;
+        p.d = nd;
+        p.depth += 1;
+      } else {
+        finish_sample<true>(P, Q, p, st);
+        finished = true;
+      }
+    }
+    claim_and_append(P, Q, next_count, out, home, p, cont, finished);
   }
   block_stats(Q.stats, st, ST_COUNT);
 }

@@ -159,14 +159,16 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   s->stat_blocks = 0;
   // Region size: block b appends to region b % NSUB.  Live records never
   // exceed `capacity`; each block takes grid-stride chunks of 256, so one
-  // block handles at most (capacity / (blocks * 256) + 1) * 256 entries and
-  // the ceil(blocks / NSUB) blocks of a residue at most
-  // capacity / NSUB + capacity / blocks + ceil(blocks / NSUB) * 256.
+  // block of a grid of G handles at most t / G + 256 of t entries and the
+  // ceil(G / NSUB) blocks of a residue at most t / NSUB + t / G +
+  // ceil(G / NSUB) * 256.  Up to two kernels append to the same ray queue,
+  // with stat_blocks / 4 <= G <= stat_blocks, and their t add up to at most
+  // `capacity`.
   const size_t per_residue = (stat_blocks + NSUB - 1) / NSUB;
-  const size_t region = capacity / NSUB + capacity / stat_blocks + per_residue * 256 + 256;
+  const size_t region = capacity / NSUB + 2 * (4 * static_cast<size_t>(capacity) / stat_blocks + per_residue * 256 + 256);
   const size_t n = region * NSUB;
-  // 2 ray queues x 4 planes + 3 material queues x 5 planes + 10 guide planes, 16 B per entry
-  const size_t qbytes = n * 16 * (2 * 4 + 3 * 5 + 10) + 40 * 256;  // + rounding of every take()
+  // 2 ray queues x 4 planes + 3 material queues x 5 planes, 16 B per entry
+  const size_t qbytes = n * 16 * (2 * 4 + 3 * 5) + 40 * 256;  // + rounding of every take()
   const size_t bytes = 256 + COUNTER_WORDS * sizeof(uint32_t) + stat_blocks * ST_COUNT * sizeof(unsigned long long) +
                        qbytes + res_items * 3 * sizeof(float) + 256;
   HIP_TRY(hipMalloc(&s->state, bytes));
@@ -185,7 +187,6 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
     for (int k = 0; k < 4; ++k) Q.rq[p][k] = reinterpret_cast<float4 *>(take(n * 16));
   for (int c = 0; c < 3; ++c)
     for (int k = 0; k < 5; ++k) Q.mq[c][k] = reinterpret_cast<float4 *>(take(n * 16));
-  for (int k = 0; k < 10; ++k) Q.gq[k] = reinterpret_cast<float4 *>(take(n * 16));
   Q.res = reinterpret_cast<float *>(take(res_items * 3 * sizeof(float)));
   s->capacity = capacity;
   s->res_items = res_items;
@@ -202,6 +203,7 @@ struct tpt_guide {
   float *adam_m = nullptr, *adam_v = nullptr;
   uint32_t *tiles = nullptr;   // G_TILES x 128 words
   uint32_t *ttiles = nullptr;  // G_TT_TILES x 128 words
+  uint32_t *xtiles = nullptr;  // inference tiles, GX_TILES x 256 words
   float *partial = nullptr;    // train blocks x G_PARTIAL
   float *scratch = nullptr;    // gradient / loss staging (G_NPARAM_PAD + 2)
   int train_blocks = 0;
@@ -215,7 +217,8 @@ namespace {
 
 int guide_pack(tpt_guide *g, hipStream_t stream) {
   hipLaunchKernelGGL(k_guide_pack, dim3((G_B1 + 255) / 256), dim3(256), 0, stream, g->params,
-                     reinterpret_cast<uint16_t *>(g->tiles), reinterpret_cast<uint16_t *>(g->ttiles));
+                     reinterpret_cast<uint16_t *>(g->tiles), reinterpret_cast<uint16_t *>(g->ttiles),
+                     reinterpret_cast<uint16_t *>(g->xtiles));
   HIP_TRY(hipGetLastError());
   return TPT_OK;
 }
@@ -404,6 +407,8 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
   if (cap > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
   if (cap > pass_items) cap = static_cast<uint32_t>(pass_items);
   const uint32_t grid = static_cast<uint32_t>(s->n_cu) * 8;
+  // k_guide_shade holds 51 KB of LDS (weights + output table): 3 blocks per CU
+  const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * 3;
   int rc = ensure_state(s, cap, pass_items, grid);
   if (rc != TPT_OK) return rc;
 
@@ -419,8 +424,9 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
   P.sumsq = d_sumsq;
   const scene_view S = s->view();
   const render_state &Q = s->Q;
+  P.guide_verts = guide ? (desc->guide_vertices ? desc->guide_vertices : 2u) : ~0u;
   if (guide) {
-    P.g_tiles = guide->tiles;
+    P.g_tiles = guide->xtiles;
     P.g_params = guide->params;
     for (int k = 0; k < 6; ++k) P.bounds[k] = s->bounds[k];
     P.rec_stride = desc->guiding;
@@ -483,12 +489,13 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
         hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
       else
         hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
-      if (guide) hipLaunchKernelGGL(k_guide_queue, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
       if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 1], stream));
-      if (guide)
+      if (guide) {
         hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
-      else
+        hipLaunchKernelGGL(k_guide_shade, dim3(guide_grid), dim3(256), 0, stream, S, P, Q, parity);
+      } else {
         hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
+      }
       if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 2], stream));
       ++batch;
       if ((it % poll) == poll - 1) {
@@ -587,6 +594,8 @@ int tpt_guide_create(uint64_t seed, tpt_guide **out) {
   HIP_TRY(hipMalloc(&g->adam_v, np * sizeof(float)));
   HIP_TRY(hipMalloc(&g->tiles, G_TILES * G_TILE_WORDS * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&g->ttiles, G_TT_TILES * G_TILE_WORDS * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&g->xtiles, GX_TILES * GX_TILE_WORDS * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(g->xtiles, 0, GX_TILES * GX_TILE_WORDS * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&g->partial, static_cast<size_t>(g->train_blocks) * G_PARTIAL * sizeof(float)));
   HIP_TRY(hipMalloc(&g->scratch, (G_NPARAM_PAD + 2) * sizeof(float)));
   HIP_TRY(hipMemcpy(g->params, h.data(), np * sizeof(float), hipMemcpyHostToDevice));
@@ -609,6 +618,7 @@ int tpt_guide_destroy(tpt_guide *g) {
   (void)hipFree(g->adam_v);
   (void)hipFree(g->tiles);
   (void)hipFree(g->ttiles);
+  (void)hipFree(g->xtiles);
   (void)hipFree(g->partial);
   (void)hipFree(g->scratch);
   (void)hipFree(g->rec_side);
@@ -644,8 +654,8 @@ int tpt_guide_infer(tpt_guide *g, const float *d_in, uint64_t n, float *d_out, v
   if (!g || (n && (!d_in || !d_out))) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_infer: null argument");
   if (n == 0) return TPT_OK;
   const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((n + 63) / 64, static_cast<uint64_t>(g->n_cu) * 8));
-  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), g->tiles, g->params,
-                     d_in, n, d_out);
+  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), g->xtiles,
+                     g->params, d_in, n, d_out);
   HIP_TRY(hipGetLastError());
   return TPT_OK;
 }
