@@ -15,6 +15,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -40,7 +42,77 @@ def parse():
     p.add_argument("--cpu-spp", type=int, default=256, help="spp of the bounded CPU baseline sample (about 15 s)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--scene", default="cornell555")
+    p.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
+    p.add_argument("--ref-spp", type=int, default=65536, help="spp of the converged plain render RMSE is taken against")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run one 256 MiB device copy first (the known-size kernel for HBM counter scaling)")
     return p.parse_args()
+
+
+RMSE_TARGET = 0.01
+# Online guided schedule: 4 training passes of 4 spp, each recording every 8th
+# path (2 diffuse vertices) and followed by 32 Adam steps of 2^18 rows; then
+# passes of 16 spp with the trained guide.
+GUIDED_SCHEDULE = dict(train_passes=4, pass_spp=4, record_stride=8, iters=32, batch=1 << 18, lr=1e-3)
+SNAPSHOT_SPP = 16
+
+
+def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=SNAPSHOT_SPP, max_spp=8192):
+    """Progressive snapshots: RMSE of the display image after each pass, with
+    the GPU time of the passes (render + training; RMSE evaluation excluded).
+    Returns the points and the interpolated time where RMSE crosses the target
+    (RMSE^2 linear in 1/t between the bracketing snapshots)."""
+    t, pts, k = 0.0, [], 0
+    while True:
+        train = k < train_passes
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        r.add(pass_spp if train else SNAPSHOT_SPP, train=train)
+        b.record()
+        torch.cuda.synchronize()
+        t += a.elapsed_time(b) / 1e3
+        k += 1
+        e = tpt.display_rmse(r.sum, r.spp, ref)
+        pts.append((r.spp, t, e))
+        if (e <= RMSE_TARGET and k >= train_passes) or r.spp >= max_spp:
+            break
+    for (s0, t0, e0), (s1, t1, e1) in zip(pts, pts[1:]):
+        if e0 > RMSE_TARGET >= e1:
+            x0, x1, y0, y1 = 1 / t0, 1 / t1, e0 * e0, e1 * e1
+            return pts, 1 / (x0 + (RMSE_TARGET ** 2 - y0) * (x1 - x0) / (y1 - y0))
+    return pts, None
+
+
+def time_to_rmse(tpt, torch, sc, ref_spp, cpu_rate):
+    W, H = sc.host.info.width, sc.host.info.height
+    ref_sum, ref_sq, _ = sc.render(spp=ref_spp, seed=987654321)
+    ref = torch.clamp(torch.sqrt(ref_sum / ref_spp), 0.0, 1.0)
+    # the reference image's own error (delta method on its per-pixel variance)
+    m = ref_sum / ref_spp
+    var = torch.clamp(ref_sq / ref_spp - m * m, min=0.0)
+    live = (m > 0) & (torch.sqrt(m) < 1)
+    floor = torch.sqrt(torch.where(live, var / (4 * m.clamp(min=1e-12)) / ref_spp, torch.zeros_like(m)).mean()).item()
+    del ref_sq, var, m
+    plain_pts, t_plain = rmse_curve(tpt, torch, tpt.ProgressiveRender(sc, seed=1), ref)
+    g = tpt.Guide(seed=1)
+    sch = GUIDED_SCHEDULE
+    rg = tpt.ProgressiveRender(sc, guide=g, seed=1, record_stride=sch["record_stride"], iters=sch["iters"],
+                               batch=sch["batch"], lr=sch["lr"])
+    guided_pts, t_guided = rmse_curve(tpt, torch, rg, ref, train_passes=sch["train_passes"],
+                                      pass_spp=sch["pass_spp"])
+    alpha = float(1 / (1 + np.exp(-g.get_params()[-1])))
+    spp_plain = plain_pts[-1][0]
+    out = {"target": RMSE_TARGET, "on": "clamp(sqrt(mean), 0, 1), all pixels and channels",
+           "against": f"plain GPU render, {ref_spp} spp", "reference_rmse_floor": round(floor, 5),
+           "plain_s": round(t_plain, 4) if t_plain else None, "plain_spp": spp_plain,
+           "guided_s": round(t_guided, 4) if t_guided else None, "guided_spp": guided_pts[-1][0],
+           "guided_speedup": round(t_plain / t_guided, 3) if t_plain and t_guided else None,
+           "guided_schedule": sch, "guided_mix_weight": round(alpha, 3),
+           "snapshots_every_spp": SNAPSHOT_SPP}
+    if cpu_rate:
+        # same estimator, so the same samples per pixel; at the CPU's measured rate
+        out["cpu_s_estimated"] = round(spp_plain * W * H / (cpu_rate * 1e6), 1)
+    return out
 
 
 def cpu_baseline(scene, w, h, spp, depth):
@@ -81,6 +153,12 @@ def main():
     import torch
 
     tpt = importlib.import_module("tiny-path-tracer_amd")
+    if args.copy_calibration:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     hs = tpt.HostScene(args.scene)
     info = hs.info
     W, H, depth = info.width, info.height, info.max_depth
@@ -115,6 +193,15 @@ def main():
     else:
         kname, kbytes, kms, nl = "k_shade", shade_bytes, totals["shade_ms"], totals["shade_launches"]
     achieved = kbytes / (kms / 1e3) / 1e9 if kms > 0 else 0.0
+    # measured HBM bytes per segment of this kernel (two rocprofv3 counter passes,
+    # scaled by a known-size copy): profiles/cornell555_1200_plain_traffic.md
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"{args.scene}_{W}_plain_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tk = json.load(f)["kernels"].get(kname)
+        if tk:
+            traffic = round(tk["hbm_bytes_per_segment"] * segs / max(nl, 1))
     out = {
         "metric": "Msamples/s (plain GPU tracer, Cornell 1200^2, 2048 spp, depth 50)",
         "value": round(value, 3),
@@ -131,7 +218,7 @@ def main():
         "config": {"workload": f"{args.scene} {W}x{H}, {spp} spp, max depth {depth}, plain estimator",
                    "width": W, "height": H, "spp": spp, "max_depth": depth},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "avg_launch_ms": round(kms / max(nl, 1), 4),
                      "bytes_per_launch": round(kbytes / max(nl, 1))},
         "detail": {"Mrays_per_s": round(segs / dt / 1e6, 2), "segments_per_path": round(segs / max(paths, 1), 3),
@@ -142,6 +229,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.scene, W, H, args.cpu_spp, depth)
         out["detail"]["x_host_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    if not args.no_rmse:
+        del sum_buf, sq_buf
+        out["time_to_rmse"] = time_to_rmse(tpt, torch, sc, args.ref_spp,
+                                           out.get("cpu_baseline", {}).get("value"))
     print(json.dumps(out), flush=True)
 
 

@@ -334,6 +334,59 @@ class Guide:
         return out
 
 
+class ProgressiveRender:
+    """A frame refined pass by pass, as the reference's progressive "bonus"
+    snapshots (main.cpp:111-133): every pass adds samples [spp, spp + n) to
+    the same sums, so any prefix of passes is an unbiased estimate.
+
+    With a Guide, passes run the guided estimator; a pass with train=True also
+    records training rows from every record_stride-th path and is followed by
+    `iters` Adam steps of `batch` rows on them (the online training loop).
+    Every sample is kept, including those of the early, barely trained passes:
+    each is an unbiased sample of the same integral."""
+
+    def __init__(self, scene, guide=None, seed=1, record_stride=8, iters=16, batch=1 << 18, lr=1e-3,
+                 guide_vertices=2, width=None, height=None, camera=None):
+        torch = _torch()
+        info = scene.host.info
+        self.scene, self.guide, self.seed = scene, guide, seed
+        self.width, self.height = width or info.width, height or info.height
+        self.camera = camera
+        self.record_stride, self.iters, self.batch, self.lr = record_stride, iters, batch, lr
+        self.guide_vertices = guide_vertices
+        shape = (self.height, self.width, 3)
+        self.sum = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        self.sumsq = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        self.spp = 0
+        self.train_loss = None
+
+    def add(self, spp, train=False):
+        """Renders spp more samples per pixel (train: record rows and train the
+        guide afterwards).  Returns the render stats."""
+        if train and self.guide is None:
+            raise TptError("training needs a Guide")
+        _, _, st = self.scene.render(width=self.width, height=self.height, spp=spp, seed=self.seed,
+                                     sample_offset=self.spp, sum_buf=self.sum, sumsq_buf=self.sumsq,
+                                     camera=self.camera, guide=self.guide,
+                                     guiding=self.record_stride if train else 0,
+                                     guide_vertices=self.guide_vertices)
+        self.spp += spp
+        if train:
+            self.train_loss = self.guide.train(None, n_batch=self.batch, iters=self.iters, lr=self.lr)
+        return st
+
+    def mean(self):
+        return self.sum / max(self.spp, 1)
+
+
+def display_rmse(sum_buf, spp, reference_display):
+    """RMSE over all pixels and channels of clamp(sqrt(mean), 0, 1), the
+    reference's display transform (main.cpp:135-136), against a display image."""
+    torch = _torch()
+    d = torch.clamp(torch.sqrt(sum_buf / spp), 0.0, 1.0)
+    return torch.sqrt(torch.mean((d - reference_display) ** 2)).item()
+
+
 def tonemap_ppm(mean_rgb):
     """The reference's output transform (main.cpp:135-142, 176-189): sqrt,
     int(255.99f·c), clamp [0,255], rows written top first.  mean_rgb: float
