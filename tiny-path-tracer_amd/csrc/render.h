@@ -48,8 +48,8 @@ __host__ __device__ constexpr uint32_t cur_at(uint32_t k) { return (8 * NSUB + k
 constexpr uint32_t COUNTER_WORDS = 9 * NSUB * CSTRIDE;
 
 struct render_state {
-  float4 *rq[2][4];    // ray queues by parity, 4 planes; NSUB regions of `region` entries
-  float4 *mq[3][5];    // material queues, 5 planes, same regions
+  float4 *rq[2][3];    // ray queues by parity, 3 planes (path record); NSUB regions of `region` entries
+  float4 *mq[3][4];    // material queues: the path record + the hit, same regions
   uint32_t region;     // entries per region
   uint32_t *counters;  // COUNTER_WORDS
   float *res;          // 3 floats per item of the pass
@@ -186,23 +186,31 @@ __device__ __forceinline__ void block_stats(unsigned long long *stats, const uin
   if (threadIdx.x < static_cast<uint32_t>(n)) stats[blockIdx.x * ST_COUNT + threadIdx.x] += acc[threadIdx.x];
 }
 
+// Path record, 48 bytes in 3 planes.  A live path's radiance is 0 or NaN per
+// channel: light is only added by the event that ends the path (an emitter or
+// a miss), and the reference's `emitted + ...` form turns a channel NaN when a
+// non-finite throughput meets a zero term.  So only its NaN bits are stored
+// (bits 16..18 next to the diffuse-vertex count).
 __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
+  const uint32_t nan_bits = (p.rad.x != p.rad.x ? 1u : 0u) | (p.rad.y != p.rad.y ? 2u : 0u) |
+                            (p.rad.z != p.rad.z ? 4u : 0u);
   q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
   q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth));
-  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff));
-  q[3][at] = make_float4(p.rad.x, p.rad.y, p.rad.z, 0.0f);
+  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16)));
 }
 
 __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
-  const float4 a = q[0][at], b = q[1][at], c = q[2][at], d = q[3][at];
+  const float4 a = q[0][at], b = q[1][at], c = q[2][at];
   path p;
   p.o = v3{a.x, a.y, a.z};
   p.item = __float_as_uint(a.w);
   p.d = v3{b.x, b.y, b.z};
   p.depth = __float_as_uint(b.w);
   p.thr = v3{c.x, c.y, c.z};
-  p.ndiff = __float_as_uint(c.w);
-  p.rad = v3{d.x, d.y, d.z};
+  const uint32_t w = __float_as_uint(c.w);
+  p.ndiff = w & 0xFFFFu;
+  const float qnan = __uint_as_float(0x7FC00000u);
+  p.rad = v3{(w & 0x10000u) ? qnan : 0.0f, (w & 0x20000u) ? qnan : 0.0f, (w & 0x40000u) ? qnan : 0.0f};
   return p;
 }
 
@@ -313,7 +321,7 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
       const uint32_t at = home * Q.region + wave_append(Q.counters + mc_at(parity, k, home), cls == k);
       if (cls == k) {
         store_ray(Q.mq[k], at, p);
-        Q.mq[k][4][at] = make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
+        Q.mq[k][3][at] = make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
       }
     }
   }
@@ -415,7 +423,7 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
       const uint32_t cls = C0 + r / NSUB;
       const uint32_t off = (r % NSUB) * Q.region + (qi - pre[r]);
       p = load_ray(Q.mq[cls], off);
-      const float4 hw = Q.mq[cls][4][off];
+      const float4 hw = Q.mq[cls][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       if (h.slot == kNone) {
         // miss: black (src/utils.cc:85-86) or the sky extension
@@ -495,7 +503,7 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
           p.thr = p.thr * w;
           cont = true;
-          p.ndiff += 1;
+          p.ndiff += p.ndiff < 0xFFFFu ? 1u : 0u;  // 16 bits in the record
         }
         if (cont) {
           p.o = pt;
@@ -557,7 +565,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const uint32_t r = find_region(pre, NSUB, qi);
       off = r * Q.region + (qi - pre[r]);
       const path p = load_ray(Q.mq[Q_DIFFUSE], off);
-      const float4 hw = Q.mq[Q_DIFFUSE][4][off];
+      const float4 hw = Q.mq[Q_DIFFUSE][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       v3 pt, n;
       float u, v;
@@ -604,7 +612,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     path p{};
     if (valid) {
       p = load_ray(Q.mq[Q_DIFFUSE], off);
-      const float4 hw = Q.mq[Q_DIFFUSE][4][off];
+      const float4 hw = Q.mq[Q_DIFFUSE][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       v3 pt, n;
       float u, v;
@@ -650,7 +658,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         sd[4] = make_float4(tp.x, tp.y, tp.z, 0.0f);
       }
       p.thr = p.thr * w;
-      p.ndiff += 1;
+      p.ndiff += p.ndiff < 0xFFFFu ? 1u : 0u;  // 16 bits in the record
       // Below the surface and off the light the integrand is zero (the
       // reference sampler never goes there): the path ends with what it has.
       cont = spdf > 0.0f && q > 0.0f;

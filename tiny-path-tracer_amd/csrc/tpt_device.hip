@@ -167,8 +167,8 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   const size_t per_residue = (stat_blocks + NSUB - 1) / NSUB;
   const size_t region = capacity / NSUB + 2 * (4 * static_cast<size_t>(capacity) / stat_blocks + per_residue * 256 + 256);
   const size_t n = region * NSUB;
-  // 2 ray queues x 4 planes + 3 material queues x 5 planes, 16 B per entry
-  const size_t qbytes = n * 16 * (2 * 4 + 3 * 5) + 40 * 256;  // + rounding of every take()
+  // 2 ray queues x 3 planes + 3 material queues x 4 planes, 16 B per entry
+  const size_t qbytes = n * 16 * (2 * 3 + 3 * 4) + 40 * 256;  // + rounding of every take()
   const size_t bytes = 256 + COUNTER_WORDS * sizeof(uint32_t) + stat_blocks * ST_COUNT * sizeof(unsigned long long) +
                        qbytes + res_items * 3 * sizeof(float) + 256;
   HIP_TRY(hipMalloc(&s->state, bytes));
@@ -184,9 +184,9 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   Q.counters = reinterpret_cast<uint32_t *>(take(COUNTER_WORDS * sizeof(uint32_t)));
   Q.stats = reinterpret_cast<unsigned long long *>(take(stat_blocks * ST_COUNT * sizeof(unsigned long long)));
   for (int p = 0; p < 2; ++p)
-    for (int k = 0; k < 4; ++k) Q.rq[p][k] = reinterpret_cast<float4 *>(take(n * 16));
+    for (int k = 0; k < 3; ++k) Q.rq[p][k] = reinterpret_cast<float4 *>(take(n * 16));
   for (int c = 0; c < 3; ++c)
-    for (int k = 0; k < 5; ++k) Q.mq[c][k] = reinterpret_cast<float4 *>(take(n * 16));
+    for (int k = 0; k < 4; ++k) Q.mq[c][k] = reinterpret_cast<float4 *>(take(n * 16));
   Q.res = reinterpret_cast<float *>(take(res_items * 3 * sizeof(float)));
   s->capacity = capacity;
   s->res_items = res_items;
