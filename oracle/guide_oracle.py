@@ -137,7 +137,9 @@ def gradient(p, rows, emulate=True):
     gr = np.zeros(NPARAM + 1)
     h1, h2, h3, x = act["h1"], act["h2"], act["h3"], act["x"]
     d_b = r(dout) if emulate else dout
-    gr[B4:NPARAM] = dout.sum(axis=0)
+    # b4: the device sums the bf16 output deltas (the operand it already has
+    # in the weight-gradient layout); b1..b3 sum the fp32 deltas
+    gr[B4:NPARAM] = d_b.sum(axis=0)
     gr[W4:B1] = (d_b.T @ (r(h3) if emulate else h3)).reshape(-1)
     dz3 = (d_b @ r(w4)) * (h3 > 0) if emulate else (dout @ w4) * (h3 > 0)
     gr[B3:B4] = dz3.sum(axis=0)

@@ -25,11 +25,6 @@
 namespace tptd {
 
 constexpr int G_IN = 42, G_IN_PAD = 48, G_HID = 64, G_OUT = 40, G_OUT_PAD = 48, G_LOBES = 8;
-// bf16 weight tiles (16x16, 512 B) in A-operand order, layer by layer:
-// W1: 4 M-tiles x 3 K-tiles, W2, W3: 4 x 4, W4: 3 x 4.
-constexpr int G_TILES1 = 12, G_TILES2 = 16, G_TILES3 = 16, G_TILES4 = 12;
-constexpr int G_TILES = G_TILES1 + G_TILES2 + G_TILES3 + G_TILES4;  // 56
-constexpr int G_TILE_WORDS = 128;                                     // 512 B = 128 uint32
 constexpr int G_BIAS = G_HID * 3 + G_OUT_PAD;                          // 240 floats
 // fp32 master parameters, padded: W1 64x48, W2 64x64, W3 64x64, W4 48x64, then biases.
 constexpr int G_W1 = 0, G_W2 = G_W1 + 64 * 48, G_W3 = G_W2 + 64 * 64, G_W4 = G_W3 + 64 * 64;
@@ -190,73 +185,6 @@ __device__ __forceinline__ void guide_forward_x32(const bf16x8 xb_in[2], const u
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[mt][r] += bias[192 + 16 * mt + 4 * fg + r];
   }
-}
-
-// Forward pass for 16 samples held by one wave: lane l works on sample l%16
-// and feature group l/16.  `in9` = this lane's sample inputs (p̂, n, wo);
-// `wt` = LDS weight tiles (A-operand order), `bias` = LDS biases.  Returns the
-// output D tiles: out[mt][r] = o[16·mt + 4·(l/16) + r] of sample l%16.  If
-// `acts` is non-null the post-ReLU hidden activations are kept (for backward):
-// acts[layer][mt][r].
-__device__ __forceinline__ void guide_forward16(const float *in9, const uint32_t *wt, const float *bias,
-                                                f32x4 out[3], f32x4 (*acts)[4]) {
-  const int l = __lane_id();
-  const int fg = l >> 4;
-  bf16x4 x[4];
-#pragma unroll
-  for (int kt = 0; kt < 3; ++kt) {
-    const int f = 16 * kt + 4 * fg;
-    x[kt] = pack4(guide_feature(f, in9), guide_feature(f + 1, in9), guide_feature(f + 2, in9),
-                  guide_feature(f + 3, in9));
-  }
-  const bf16x4 *tiles = reinterpret_cast<const bf16x4 *>(wt);
-  int tile = 0;
-  // three hidden layers
-#pragma unroll
-  for (int layer = 0; layer < 3; ++layer) {
-    const int kts = layer == 0 ? 3 : 4;
-    f32x4 acc[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) {
-      if (kt < kts) {
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tiles[(tile + mt * kts + kt) * 64 + l], x[kt], acc[mt], 0,
-                                                             0, 0);
-      }
-    }
-    tile += 4 * kts;
-    const float *b = bias + 64 * layer;
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      f32x4 h;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float z = acc[mt][r] + b[16 * mt + 4 * fg + r];
-        h[r] = z > 0.0f ? z : 0.0f;
-      }
-      if (acts) acts[layer][mt] = h;
-      x[mt] = pack4(h[0], h[1], h[2], h[3]);
-    }
-    // keep the next layer's weight loads from being hoisted above this one
-    // (they would hold up to 32 more registers)
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  // output layer: 3 M-tiles x 4 K-tiles
-#pragma unroll
-  for (int mt = 0; mt < 3; ++mt) out[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-  for (int kt = 0; kt < 4; ++kt)
-#pragma unroll
-    for (int mt = 0; mt < 3; ++mt)
-      out[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tiles[(tile + mt * 4 + kt) * 64 + l], x[kt], out[mt], 0, 0, 0);
-  const float *b4 = bias + 192;
-#pragma unroll
-  for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) out[mt][r] += b4[16 * mt + 4 * fg + r];
 }
 
 // ---------------------------------------------------------------- vMF mix ---

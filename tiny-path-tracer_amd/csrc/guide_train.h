@@ -1,18 +1,29 @@
-// Guiding network kernels: weight packing, inference, fused forward/backward
-// training step (BF16 MFMA, FP32 accumulation) and the reduce + Adam update.
+// Guiding network kernels: weight packing, inference, the fused
+// forward/backward training step (BF16 MFMA, FP32 accumulation) and the
+// gradient reduction + Adam update.
 //
-// Training step, per block of 4 waves and 64 samples (16 per wave):
-//   1. forward (guide.h layout: activations stay in the MFMA D/B layout) and
-//      copies of x, h1, h2, h3 as bf16 into LDS rows [sample][feature];
-//   2. one lane per sample evaluates the vMF mixture at the stored direction
-//      and the gradient of -(c/n) log g with respect to the 40 outputs;
-//   3. backward-data through W4ᵀ, W3ᵀ, W2ᵀ with MFMA on pre-transposed weight
-//      tiles, gated by ReLU'; the gradients dz go to LDS as bf16 rows;
-//   4. weight gradients dW = dz · hᵀ over the 64 samples: the 56 dW tiles are
-//      split 14 per wave and accumulated in registers across steps; the
-//      transposed operands are read straight from the LDS rows.
-// Each block then writes its partial gradient; k_guide_adam sums the partials
-// and applies Adam to the fp32 master weights, re-packing the bf16 tiles.
+// Training step (k_guide_train).  Each wave works alone on 16 samples at a
+// time (lane l: sample l%16, lane group g = l/16), with no LDS traffic and no
+// block barrier inside its loop:
+//   1. forward in the inference layout (16x16x32, permuted K order, guide.h):
+//      the D tiles, "sample-major" (a lane holds neurons 16mt + 4g + r of one
+//      sample), feed the next layer directly.  The output rows are permuted so
+//      that lane group g holds lobes 2g and 2g+1 of its sample;
+//   2. the loss gradient: each lane evaluates its two lobes; the softmax and
+//      the mixture density are completed across the sample's four lanes by
+//      shuffles, so all 64 lanes work;
+//   3. backward data, sample-major, through W4ᵀ and W3ᵀ (transposed tiles in
+//      the same permuted layout) to chain the deltas;
+//   4. the weight gradients need "neuron-major" operands (a lane holds one
+//      neuron of 4 samples).  Swapping the MFMA operands (B·A instead of A·B)
+//      gives exactly that layout from the same registers: activations from
+//      (activation, W) products, deltas from (delta, Wᵀ) products, and the
+//      inputs and output deltas from products with identity tiles;
+//   5. dW += δ·hᵀ over the 16 samples with v_mfma_f32_16x16x16_bf16, the 56
+//      tiles accumulated in registers for the whole launch.
+// The four waves of a block then sum their tiles through LDS into one block
+// partial; k_guide_reduce adds the partials into an accumulator and
+// k_guide_adam turns it into the gradient or the Adam update (and clears it).
 #ifndef TPT_CSRC_GUIDE_TRAIN_H
 #define TPT_CSRC_GUIDE_TRAIN_H
 
@@ -20,17 +31,30 @@
 
 namespace tptd {
 
-constexpr int G_TT_TILES = 12 + 16 + 16;  // transposed tiles for W4ᵀ, W3ᵀ, W2ᵀ
-constexpr int G_PARTIAL = G_NPARAM_PAD + 4;  // + alpha grad, loss sum, count, pad
-// LDS row layout for one step: [64 samples][480 features] bf16
-constexpr int A_X = 0, A_H1 = 48, A_H2 = 112, A_H3 = 176, A_DO = 240, A_DZ3 = 288, A_DZ2 = 352, A_DZ1 = 416,
-              A_W = 480;
+constexpr int G_PARTIAL = G_NPARAM_PAD + 4;  // + mix-logit gradient, loss sum, 2 pad
+// Training tile set (1 KB tiles, 16x16x32 layout): forward W1..W4 (W4 rows in
+// lobe order), transposed W4ᵀ, W3ᵀ, W2ᵀ, and the identity tiles.
+constexpr int GR_FWD = 30, GR_BWD = 24, GR_ID = 12, GR_TILES = GR_FWD + GR_BWD + GR_ID;
+constexpr int GR_BWD_BASE = GR_FWD;           // W4ᵀ, then W3ᵀ, then W2ᵀ: 8 tiles each
+constexpr int GR_IDX_BASE = GR_FWD + GR_BWD;  // inputs: 3 column tiles x 2 K steps
+constexpr int GR_IDO_BASE = GR_IDX_BASE + 6;  // outputs: 3 column tiles x 2 K steps
 
-// Layer geometry: fp32 offset, rows (out), padded cols (in), M/K tiles, tile base.
-__host__ __device__ inline void layer_geom(int L, int *off, int *rows, int *cols, int *mts, int *kts, int *base) {
+// Output row order of the training forward: physical row p = 16 mt + 4 g + r
+// holds output u = 4 mt + r (u < 10) of lane group g, i.e. logical output
+// 10 g + u (lobes 2g, 2g+1); rows with u >= 10 are padding.
+__host__ __device__ inline int out_phys(int o) {
+  const int g = o / 10, u = o % 10;
+  return 16 * (u / 4) + 4 * g + (u % 4);
+}
+__host__ __device__ inline int out_logical(int p) {
+  const int u = 4 * (p / 16) + (p % 4);
+  return u < 10 ? 10 * ((p % 16) / 4) + u : -1;
+}
+
+// Layer geometry: fp32 offset, rows (out), padded cols (in).
+__host__ __device__ inline void layer_geom(int L, int *off, int *rows, int *cols) {
   const int o[4] = {G_W1, G_W2, G_W3, G_W4}, r[4] = {64, 64, 64, 48}, c[4] = {48, 64, 64, 64};
-  const int b[4] = {0, 12, 28, 44};
-  *off = o[L], *rows = r[L], *cols = c[L], *mts = r[L] / 16, *kts = c[L] / 16, *base = b[L];
+  *off = o[L], *rows = r[L], *cols = c[L];
 }
 
 // True where a padded-layout parameter is real (not padding).
@@ -43,42 +67,58 @@ __host__ __device__ inline bool param_live(int p) {
   return true;
 }
 
-// bf16 tiles from fp32 master: training forward tiles (A = W), transposed
-// tiles (A = Wᵀ, for layers 4, 3, 2) and the inference tiles (16x16x32
-// layout, permuted K order), one thread per (layer, out, in).  Inference-tile
-// slots that hold no weight are never written (they stay zero).
-__global__ void k_guide_pack(const float *__restrict__ w, uint16_t *__restrict__ tiles, uint16_t *__restrict__ ttiles,
-                             uint16_t *__restrict__ xtiles) {
+// K slot (s, g, q) of a 16x16x32 operand that holds hidden/output neuron n in
+// the permuted order n = 16 (2 s + q/4) + 4 g + q%4 (a lane's own D tiles).
+__host__ __device__ inline void neuron_kslot(int n, int *s, int *g, int *q) {
+  const int mt = n / 16;
+  *s = mt / 2, *g = (n % 16) / 4, *q = 4 * (mt % 2) + (n % 4);
+}
+
+// bf16 tiles from the fp32 master weights, one thread per (layer, out, in):
+// the inference tiles (natural output order) and the training tiles.  Slots
+// that hold no weight are never written (they stay zero).  Threads below
+// 12 * 512 also write the identity tiles.
+__global__ void k_guide_pack(const float *__restrict__ w, uint16_t *__restrict__ xtiles, uint16_t *__restrict__ rtiles) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G_B1) return;
-  int L = t < G_W2 ? 0 : (t < G_W3 ? 1 : (t < G_W4 ? 2 : 3));
-  int off, rows, cols, mts, kts, base;
-  layer_geom(L, &off, &rows, &cols, &mts, &kts, &base);
-  const int o = (t - off) / cols, i = (t - off) % cols;
-  const uint16_t v = to_bf16(param_live(t) ? w[t] : 0.0f);
-  // forward: A[o][i], tile (o/16, i/16), lane (o%16) + 16*((i%16)/4), reg i%4
-  const int tile = base + (o / 16) * kts + (i / 16);
-  tiles[(tile * 64 + (o % 16) + 16 * ((i % 16) / 4)) * 4 + (i % 4)] = v;
-  if (L >= 1) {
-    // transposed: A[i][o], tiles M = cols/16, K = rows/16; order W4ᵀ, W3ᵀ, W2ᵀ
-    const int tb = L == 3 ? 0 : (L == 2 ? 12 : 28);
-    const int tkts = rows / 16;
-    const int tt = tb + (i / 16) * tkts + (o / 16);
-    ttiles[(tt * 64 + (i % 16) + 16 * ((o % 16) / 4)) * 4 + (o % 4)] = v;
+  if (t < GR_ID * 512) {
+    const int tile = t / 512, lane = (t / 8) % 64, q = t % 8, j = lane % 16, g = lane / 16;
+    const int ni = (tile % 6) / 2, s = tile % 2;
+    bool one;
+    if (tile < 6) {
+      one = g == ni && j == 8 * s + q;  // input slot (g, 8 s + q) -> column 16 g + 8 s + q
+    } else {
+      one = 16 * (2 * s + q / 4) + 4 * g + q % 4 == 16 * ni + j;  // output row -> column
+    }
+    rtiles[(GR_IDX_BASE + tile) * 512 + lane * 8 + q] = one ? 0x3F80u : 0u;
   }
-  // inference: K slot (s, g, q) of the input column i
+  if (t >= G_B1) return;
+  const int L = t < G_W2 ? 0 : (t < G_W3 ? 1 : (t < G_W4 ? 2 : 3));
+  int off, rows, cols;
+  layer_geom(L, &off, &rows, &cols);
+  const int o = (t - off) / cols, i = (t - off) % cols;
+  if (!param_live(t)) return;
+  const uint16_t v = to_bf16(w[t]);
+  // input column i -> K slot
   int g, s, q;
   if (L == 0) {
-    if (!param_live(t)) return;
     int slot;
     feature_xslot(i, &g, &slot);
     s = slot / 8, q = slot % 8;
   } else {
-    const int mt = i / 16;
-    g = (i % 16) / 4, s = mt / 2, q = 4 * (mt % 2) + (i % 4);
+    neuron_kslot(i, &s, &g, &q);
   }
-  const int xt = GX_BASE[L] + (o / 16) * 2 + s;
-  xtiles[(xt * 64 + (o % 16) + 16 * g) * 8 + q] = v;
+  // inference: natural output rows
+  xtiles[((GX_BASE[L] + (o / 16) * 2 + s) * 64 + (o % 16) + 16 * g) * 8 + q] = v;
+  // training forward: W4 rows in lobe order
+  const int p = L == 3 ? out_phys(o) : o;
+  rtiles[((GX_BASE[L] + (p / 16) * 2 + s) * 64 + (p % 16) + 16 * g) * 8 + q] = v;
+  // training transposed (A = Wᵀ: rows = inputs i, K = outputs p in the permuted order)
+  if (L >= 1) {
+    int ks, kg, kq;
+    neuron_kslot(p, &ks, &kg, &kq);
+    const int tb = GR_BWD_BASE + 8 * (3 - L);  // W4ᵀ at 0, W3ᵀ at 8, W2ᵀ at 16
+    rtiles[((tb + (i / 16) * 2 + ks) * 64 + (i % 16) + 16 * kg) * 8 + kq] = v;
+  }
 }
 
 __device__ __forceinline__ void guide_stage_weights(uint32_t *dst, const uint32_t *src, int words) {
@@ -115,293 +155,344 @@ __global__ void __launch_bounds__(256) k_guide_infer(const uint32_t *__restrict_
   }
 }
 
-// Loss gradient for one sample (one lane): outputs o[40] -> dout[40] (in
-// place of o), returns c·(-log g) contribution; adds the mix-weight gradient.
-__device__ __forceinline__ float guide_loss_grad(float *o, const float *row, float inv_n, float alpha, float *dalpha) {
-  const float c = row[12];
-  const float q = row[13];
-  const v3 w{row[9], row[10], row[11]};
-  if (!(c > 0.0f) || !(c < 3.0e38f)) {
-    for (int k = 0; k < G_OUT; ++k) o[k] = 0.0f;
-    return 0.0f;
-  }
-  vmf_mix m;
-  decode_mix(o, &m);
-  float dv[G_LOBES], cs[G_LOBES];
-  float g = 0.0f;
-  for (int i = 0; i < G_LOBES; ++i) {
-    cs[i] = m.mu[i][0] * w.x + m.mu[i][1] * w.y + m.mu[i][2] * w.z;
-    dv[i] = vmf_pdf(m.kappa[i], cs[i]);
-    g += m.w[i] * dv[i];
-  }
-  g = fmaxf(g, 1e-30f);
-  const float mm = alpha * g + (1.0f - alpha) * q;
-  if (mm > 0.0f) *dalpha += -c * inv_n * (g - q) * alpha * (1.0f - alpha) / mm;
-  const float loss = -c * __logf(g);
-  const float f = -c * inv_n / g;
-  for (int i = 0; i < G_LOBES; ++i) {
-    const float a = o[5 * i], b = o[5 * i + 1], cc = o[5 * i + 2];
-    const float nrm = sqrtf(a * a + b * b + cc * cc);
-    const float kap = m.kappa[i];
-    const float lk = o[5 * i + 3];
-    const float wd = m.w[i] * dv[i];
-    // d/d raw mean: κ (ω − (μ·ω) μ) / |raw|
-    const float s = nrm > 1e-10f ? f * wd * kap / nrm : 0.0f;
-    o[5 * i] = s * (w.x - cs[i] * m.mu[i][0]);
-    o[5 * i + 1] = s * (w.y - cs[i] * m.mu[i][1]);
-    o[5 * i + 2] = s * (w.z - cs[i] * m.mu[i][2]);
-    // d/d log κ: 1 − 2κ/(e^{2κ} − 1) + κ(μ·ω − 1), zero where the clamp is active
-    const bool live = lk > G_LOGK_MIN && lk < G_LOGK_MAX;
-    o[5 * i + 3] = live ? f * wd * (1.0f - 2.0f * kap / expm1f(2.0f * kap) + kap * (cs[i] - 1.0f)) : 0.0f;
-    // d/d logit: w_i (D_i − g)
-    o[5 * i + 4] = f * m.w[i] * (dv[i] - g);
-  }
-  return loss;
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ bf16x4 lds_col4(const uint16_t *act, int s0, int col) {
-  // act[(s0 + r) * A_W + col], r = 0..3
-  bf16x4 v;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) v[r] = static_cast<short>(act[(s0 + r) * A_W + col]);
-  return v;
-}
+__device__ __forceinline__ bf16x4 pack4v(f32x4 v) { return pack4(v[0], v[1], v[2], v[3]); }
 
-// One backward-data layer: dh[i][s] = Σ_o W[o][i] dz[o][s] via transposed
-// tiles (tt_base, M = in/16 = 4 tiles, K = out/16 tiles), gated by ReLU'(h).
-__device__ __forceinline__ void guide_bwd_data(const bf16x4 *tt, int tt_base, int ktiles, const bf16x4 dz[4],
-                                               const f32x4 h[4], f32x4 out[4]) {
-  const int l = __lane_id();
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) out[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-  for (int kt = 0; kt < 4; ++kt) {
-    if (kt < ktiles) {
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt)
-        out[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tt[(tt_base + mt * ktiles + kt) * 64 + l], dz[kt], out[mt], 0,
-                                                           0, 0);
-    }
+// One lobe of a sample's mixture, as held by the lane that owns it.
+struct lobe {
+  float mu[3], nrm, kap, lk, cs, d, w;
+};
+
+// raw = (mx, my, mz, log κ); om = the sampled direction ω.
+__device__ __forceinline__ void lobe_eval(float a, float b, float c, float lk, v3 om, lobe *L) {
+  const float n2 = a * a + b * b + c * c;
+  L->nrm = sqrtf(n2);
+  if (n2 > 1e-20f) {
+    const float inv = rsqrtf(n2);
+    L->mu[0] = a * inv, L->mu[1] = b * inv, L->mu[2] = c * inv;
+  } else {
+    L->mu[0] = 0.0f, L->mu[1] = 0.0f, L->mu[2] = 1.0f;
   }
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) out[mt][r] = h[mt][r] > 0.0f ? out[mt][r] : 0.0f;
+  L->lk = lk;
+  L->kap = __expf(fminf(fmaxf(lk, G_LOGK_MIN), G_LOGK_MAX));
+  L->cs = L->mu[0] * om.x + L->mu[1] * om.y + L->mu[2] * om.z;
+  L->d = vmf_pdf(L->kap, L->cs);
 }
 
-// Adds a D-layout feature block (4 tiles or 3) summed over the 16 samples of
-// the wave into LDS bias accumulators db[off + feature].
-__device__ __forceinline__ void guide_bias_acc(float *db, int off, const f32x4 *v, int tiles) {
-  const int l = __lane_id(), fg = l >> 4;
-  for (int mt = 0; mt < tiles; ++mt) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float x = v[mt][r];
-      x += __shfl_xor(x, 1);
-      x += __shfl_xor(x, 2);
-      x += __shfl_xor(x, 4);
-      x += __shfl_xor(x, 8);
-      if ((l & 15) == 0) atomicAdd(&db[off + 16 * mt + 4 * fg + r], x);
-    }
-  }
+// The lobe's 5 output gradients of -(c/n) log g, given f = -c / (n g) and g.
+__device__ __forceinline__ void lobe_grad(const lobe &L, v3 om, float f, float g, float *o5) {
+  const float wd = L.w * L.d;
+  // d/d raw mean: κ (ω − (μ·ω) μ) / |raw|
+  const float s = L.nrm > 1e-10f ? f * wd * L.kap / L.nrm : 0.0f;
+  o5[0] = s * (om.x - L.cs * L.mu[0]);
+  o5[1] = s * (om.y - L.cs * L.mu[1]);
+  o5[2] = s * (om.z - L.cs * L.mu[2]);
+  // d/d log κ: 1 − 2κ/(e^{2κ} − 1) + κ(μ·ω − 1), zero where the clamp is active
+  const bool live = L.lk > G_LOGK_MIN && L.lk < G_LOGK_MAX;
+  o5[3] = live ? f * wd * (1.0f - 2.0f * L.kap / expm1f(2.0f * L.kap) + L.kap * (L.cs - 1.0f)) : 0.0f;
+  // d/d logit: w_i (D_i − g)
+  o5[4] = f * L.w * (L.d - g);
 }
 
-__device__ __forceinline__ void guide_store_rows(uint16_t *act, int col0, int tiles, const f32x4 *v) {
-  const int l = __lane_id(), fg = l >> 4, s = (threadIdx.x / 64) * 16 + (l & 15);
-  for (int mt = 0; mt < tiles; ++mt)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) act[s * A_W + col0 + 16 * mt + 4 * fg + r] = to_bf16(v[mt][r]);
-}
-
-// Fused forward/backward over rows [0, n).  Dynamic LDS: see k_guide_train_lds.
 constexpr size_t k_guide_train_lds() {
-  return G_TILES * 512 + G_TT_TILES * 512 + G_BIAS * 4 + 64 * A_W * 2 + 4 * 16 * 48 * 4 + G_BIAS * 4 + 64;
+  return GR_TILES * 1024 + G_BIAS * 4 + 3 * 14 * 1024 + (G_BIAS + 4) * 4;
 }
 
-__global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict__ tiles, const uint32_t *__restrict__ ttiles,
-                                                     const float *__restrict__ params, const float *__restrict__ rows,
-                                                     uint64_t n, uint64_t row0, uint64_t n_total,
-                                                     float *__restrict__ partial) {
+// Fused forward/backward over rows [0, n) (row k read at (row0 + k) % n_total).
+// 256 threads per block; each wave takes groups of 16 rows.  Writes this
+// block's partial gradient (natural parameter layout) to partial.
+__global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict__ rtiles, const float *__restrict__ params,
+                                                     const float *__restrict__ rows, uint64_t n, uint64_t row0,
+                                                     uint64_t n_total, float *__restrict__ partial) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t *wt = reinterpret_cast<uint32_t *>(lds);
-  uint32_t *wtt = wt + G_TILES * G_TILE_WORDS;
-  float *bias = reinterpret_cast<float *>(wtt + G_TT_TILES * G_TILE_WORDS);
-  uint16_t *act = reinterpret_cast<uint16_t *>(bias + G_BIAS);
-  float *outs = reinterpret_cast<float *>(act + 64 * A_W);
-  float *db = outs + 4 * 16 * 48;
-  float *scal = db + G_BIAS;  // [0] dalpha, [1] loss, [2] count
-  guide_stage_weights(wt, tiles, G_TILES * G_TILE_WORDS);
-  guide_stage_weights(wtt, ttiles, G_TT_TILES * G_TILE_WORDS);
+  float *bias = reinterpret_cast<float *>(wt + GR_TILES * 256);  // b1 b2 b3 natural, b4 in lobe order
+  f32x4 *xch = reinterpret_cast<f32x4 *>(bias + G_BIAS);         // 3 waves x 14 tiles x 64 lanes
+  float *sums = reinterpret_cast<float *>(xch + 3 * 14 * 64);    // biases, mix-logit gradient, loss
+  guide_stage_weights(wt, rtiles, GR_TILES * 256);
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) {
-    bias[i] = params[G_B1 + i];
-    db[i] = 0.0f;
+    float b = 0.0f;
+    if (i < 192) b = params[G_B1 + i];
+    else if (out_logical(i - 192) >= 0) b = params[G_B4 + out_logical(i - 192)];
+    bias[i] = b;
   }
-  if (threadIdx.x < 4) scal[threadIdx.x] = 0.0f;
+  for (int i = threadIdx.x; i < G_BIAS + 4; i += blockDim.x) sums[i] = 0.0f;
   __syncthreads();
-  const int l = __lane_id(), wave = threadIdx.x / 64, fg = l >> 4, sl = l & 15;
+  const bf16x8 *T = reinterpret_cast<const bf16x8 *>(wt);
+  const int l = __lane_id(), wave = threadIdx.x / 64, g = l >> 4, j = l & 15;
   const float inv_n = 1.0f / static_cast<float>(n);
   const float alpha = 1.0f / (1.0f + __expf(-params[G_NPARAM_PAD]));
-  const bf16x4 *T = reinterpret_cast<const bf16x4 *>(wt);
-  const bf16x4 *TT = reinterpret_cast<const bf16x4 *>(wtt);
-  // this wave's 14 dW tiles: global tile index j = 14*wave + t over the list
-  // (layer 1: 4x3, layer 2: 4x4, layer 3: 4x4, layer 4: 3x4), accumulated across steps
-  f32x4 dw[14];
+  // dW tiles: W4 3x4 (0..11), W3 4x4 (12..27), W2 4x4 (28..43), W1 4x3 (44..55)
+  f32x4 dw[56];
 #pragma unroll
-  for (int t = 0; t < 14; ++t) dw[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < 56; ++t) dw[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float db[15];  // per lane: neuron 16 t + j of b1 (0..3), b2 (4..7), b3 (8..11), b4 lobe-order rows (12..14)
+#pragma unroll
+  for (int k = 0; k < 15; ++k) db[k] = 0.0f;
   float dalpha = 0.0f, loss = 0.0f;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * 64; base < n; base += static_cast<uint64_t>(gridDim.x) * 64) {
-    const uint64_t sidx = base + wave * 16 + sl;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * 4;
+  for (uint64_t base = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * 16; base < n; base += nwaves * 16) {
+    // an opaque zero offset keeps the weight-tile loads inside the loop (hoisted
+    // out of it they would hold the whole tile set in registers)
+    int zoff = 0;
+    asm volatile("" : "+v"(zoff));
+    const bf16x8 *Tl = T + zoff;
+    const uint64_t sidx = base + j;
     const bool valid = sidx < n;
     float row[16];
-    const float *rp = rows + ((row0 + sidx) % n_total) * TPT_GUIDE_ROW;
+    const float *rp = rows + ((row0 + (valid ? sidx : 0)) % n_total) * TPT_GUIDE_ROW;
 #pragma unroll
     for (int k = 0; k < 16; ++k) row[k] = valid ? rp[k] : 0.0f;
-    // ---- forward (keeps activations)
-    bf16x4 x[4];
-    f32x4 xf[3];
-#pragma unroll
-    for (int kt = 0; kt < 3; ++kt) {
-      const int f = 16 * kt + 4 * fg;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) xf[kt][r] = guide_feature(f + r, row);
-      x[kt] = pack4(xf[kt][0], xf[kt][1], xf[kt][2], xf[kt][3]);
-    }
-    guide_store_rows(act, A_X, 3, xf);
-    f32x4 h[3][4];
-    int tile = 0;
+    // ---- 1. forward, sample-major
+    bf16x8 xa[2], a1[2], a2[2], a3[2];
+    guide_xfeatures(g, g == 0 ? row[0] : (g == 1 ? row[1] : row[2]), row + 3, xa);
+    uint32_t m2 = 0, m3 = 0;  // sample-major ReLU masks of layers 2, 3, bit 4 mt + r
 #pragma unroll
     for (int layer = 0; layer < 3; ++layer) {
-      const int kts = layer == 0 ? 3 : 4;
+      const bf16x8 *in = layer == 0 ? xa : (layer == 1 ? a1 : a2);
       f32x4 acc[4];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kt = 0; kt < 4; ++kt)
-        if (kt < kts) {
-#pragma unroll
-          for (int mt = 0; mt < 4; ++mt)
-            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T[(tile + mt * kts + kt) * 64 + l], x[kt], acc[mt], 0, 0, 0);
-        }
-      tile += 4 * kts;
-#pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
+        acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) acc[mt] = mfma32(Tl[(GX_BASE[layer] + 2 * mt + s) * 64 + l], in[s], acc[mt]);
+      }
+      uint32_t m = 0;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const float z = acc[mt][r] + bias[64 * layer + 16 * mt + 4 * fg + r];
-          h[layer][mt][r] = z > 0.0f ? z : 0.0f;
+          const float z = acc[mt][r] + bias[64 * layer + 16 * mt + 4 * g + r];
+          m |= z > 0.0f ? (1u << (4 * mt + r)) : 0u;
+          acc[mt][r] = z > 0.0f ? z : 0.0f;
         }
-        x[mt] = pack4(h[layer][mt][0], h[layer][mt][1], h[layer][mt][2], h[layer][mt][3]);
-      }
-      guide_store_rows(act, layer == 0 ? A_H1 : (layer == 1 ? A_H2 : A_H3), 4, h[layer]);
+      bf16x8 *outp = layer == 0 ? a1 : (layer == 1 ? a2 : a3);
+      outp[0] = pack8(acc[0], acc[1]);
+      outp[1] = pack8(acc[2], acc[3]);
+      if (layer == 1) m2 = m;
+      else if (layer == 2) m3 = m;
     }
     f32x4 o[3];
 #pragma unroll
-    for (int mt = 0; mt < 3; ++mt) o[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt)
-#pragma unroll
-      for (int mt = 0; mt < 3; ++mt)
-        o[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(T[(tile + mt * 4 + kt) * 64 + l], x[kt], o[mt], 0, 0, 0);
-    float *myo = outs + (wave * 16 + sl) * 48;
-#pragma unroll
-    for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) myo[16 * mt + 4 * fg + r] = o[mt][r] + bias[192 + 16 * mt + 4 * fg + r];
-    __syncthreads();
-    // ---- loss gradient, one lane per sample
-    if (fg == 0) {
-      if (valid) loss += guide_loss_grad(myo, row, inv_n, alpha, &dalpha);
-      else
-        for (int k = 0; k < 48; ++k) myo[k] = 0.0f;
-      for (int k = G_OUT; k < 48; ++k) myo[k] = 0.0f;
-    }
-    __syncthreads();
-    // ---- backward
-    f32x4 dout[3];
-    bf16x4 dz[4];
-#pragma unroll
     for (int mt = 0; mt < 3; ++mt) {
+      o[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dout[mt][r] = myo[16 * mt + 4 * fg + r];
-      dz[mt] = pack4(dout[mt][0], dout[mt][1], dout[mt][2], dout[mt][3]);
+      for (int s = 0; s < 2; ++s) o[mt] = mfma32(Tl[(GX_BASE[3] + 2 * mt + s) * 64 + l], a3[s], o[mt]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) o[mt][r] += bias[192 + 16 * mt + 4 * g + r];
     }
-    guide_store_rows(act, A_DO, 3, dout);
-    guide_bias_acc(db, 192, dout, 3);
-    f32x4 dh[4];
-    guide_bwd_data(TT, 0, 3, dz, h[2], dh);  // through W4ᵀ, gated by h3
-    guide_store_rows(act, A_DZ3, 4, dh);
-    guide_bias_acc(db, 128, dh, 4);
+    // ---- 2. loss gradient: this lane holds lobes 2g (o0.xyzw, o1.x) and 2g+1 (o1.yzw, o2.xy)
+    const v3 om{row[9], row[10], row[11]};
+    const float c = row[12], qd = row[13];
+    const bool use = valid && c > 0.0f && c < 3.0e38f;
+    lobe A, B;
+    lobe_eval(o[0][0], o[0][1], o[0][2], o[0][3], om, &A);
+    lobe_eval(o[1][1], o[1][2], o[1][3], o[2][0], om, &B);
+    const float la = o[1][0], lb = o[2][1];
+    float mx = fmaxf(la, lb);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float ea = __expf(la - mx), eb = __expf(lb - mx);
+    float z = ea + eb;
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
+    const float iz = 1.0f / z;
+    A.w = ea * iz, B.w = eb * iz;
+    float gp = A.w * A.d + B.w * B.d;
+    gp += __shfl_xor(gp, 16);
+    gp += __shfl_xor(gp, 32);
+    gp = fmaxf(gp, 1e-30f);
+    const float f = use ? -c * inv_n / gp : 0.0f;
+    float ga[5], gb[5];
+    lobe_grad(A, om, f, gp, ga);
+    lobe_grad(B, om, f, gp, gb);
+    f32x4 d[3];
+    d[0] = use ? f32x4{ga[0], ga[1], ga[2], ga[3]} : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    d[1] = use ? f32x4{ga[4], gb[0], gb[1], gb[2]} : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    d[2] = use ? f32x4{gb[3], gb[4], 0.0f, 0.0f} : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (use && g == 0) {
+      loss += -c * __logf(gp);
+      const float mm = alpha * gp + (1.0f - alpha) * qd;
+      if (mm > 0.0f) dalpha += -c * inv_n * (gp - qd) * alpha * (1.0f - alpha) / mm;
+    }
+    // ---- 3..5, layer by layer from the top, keeping few operands live:
+    // neuron-major operands (lane: neuron 16 t + j; registers: samples 4g + r)
+    // come from swapped products; dW += δ hᵀ over these 16 samples.
+    const bf16x8 dob[2] = {pack8(d[0], d[1]), pack8(d[2], f32x4{0.0f, 0.0f, 0.0f, 0.0f})};
+    // Weight layer L = 3 (W4) .. 0 (W1): its output delta δ_L and its input
+    // activations h_L, neuron-major, give dW_L += δ_L h_Lᵀ.  `dup` is δ_{L+1}
+    // sample-major (δ_3 = dout) and `dmask` the neuron-major ReLU' of the
+    // activations computed in the previous iteration (those of δ_L's layer).
+    bf16x8 dup[2] = {dob[0], dob[1]};
+    uint32_t dmask = 0;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) dz[mt] = pack4(dh[mt][0], dh[mt][1], dh[mt][2], dh[mt][3]);
-    guide_bwd_data(TT, 12, 4, dz, h[1], dh);  // through W3ᵀ, gated by h2
-    guide_store_rows(act, A_DZ2, 4, dh);
-    guide_bias_acc(db, 64, dh, 4);
+    for (int L = 3; L >= 0; --L) {
+      // h_L: L > 0 from the swapped forward of layer L-1 (input a_{L-1}), L = 0 the inputs
+      bf16x4 hn[4];
+      uint32_t nm = 0;
+      if (L > 0) {
+        const bf16x8 *hin = L == 3 ? a2 : (L == 2 ? a1 : xa);
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) dz[mt] = pack4(dh[mt][0], dh[mt][1], dh[mt][2], dh[mt][3]);
-    guide_bwd_data(TT, 28, 4, dz, h[0], dh);  // through W2ᵀ, gated by h1
-    guide_store_rows(act, A_DZ1, 4, dh);
-    guide_bias_acc(db, 0, dh, 4);
-    __syncthreads();
-    // ---- weight gradients over the block's 64 samples (4 K-steps of 16)
+        for (int t = 0; t < 4; ++t) {
+          f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int t = 0; t < 14; ++t) {
-      const int j = 14 * wave + t;
-      int L, mo, ni;
-      if (j < 12) L = 0, mo = j / 3, ni = j % 3;
-      else if (j < 28) L = 1, mo = (j - 12) / 4, ni = (j - 12) % 4;
-      else if (j < 44) L = 2, mo = (j - 28) / 4, ni = (j - 28) % 4;
-      else L = 3, mo = (j - 44) / 4, ni = (j - 44) % 4;
-      const int dzc = L == 0 ? A_DZ1 : (L == 1 ? A_DZ2 : (L == 2 ? A_DZ3 : A_DO));
-      const int inc = L == 0 ? A_X : (L == 1 ? A_H1 : (L == 2 ? A_H2 : A_H3));
+          for (int s = 0; s < 2; ++s) acc = mfma32(hin[s], Tl[(GX_BASE[L - 1] + 2 * t + s) * 64 + l], acc);
+          const float b = bias[64 * (L - 1) + 16 * t + j];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const bf16x4 a = lds_col4(act, 16 * ks + 4 * fg, dzc + 16 * mo + sl);
-        const bf16x4 b = lds_col4(act, 16 * ks + 4 * fg, inc + 16 * ni + sl);
-        dw[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, dw[t], 0, 0, 0);
+          for (int r = 0; r < 4; ++r) {
+            const float zz = acc[r] + b;
+            nm |= zz > 0.0f ? (1u << (4 * t + r)) : 0u;
+            acc[r] = zz > 0.0f ? zz : 0.0f;
+          }
+          hn[t] = pack4v(acc);
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+          f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int s = 0; s < 2; ++s) acc = mfma32(xa[s], Tl[(GR_IDX_BASE + 2 * t + s) * 64 + l], acc);
+          hn[t] = pack4v(acc);
+        }
+        hn[3] = bf16x4{0, 0, 0, 0};
       }
+      // δ_L, neuron-major: dout through the identity tiles, else
+      // (W_{L+1}ᵀ δ_{L+1}) gated by ReLU' of layer L's activations
+      bf16x4 dn[4];
+      const int nt = L == 3 ? 3 : 4;
+      const int tb = L == 3 ? GR_IDO_BASE : GR_BWD_BASE + 8 * (2 - L);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (t < nt) {
+          f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int s = 0; s < 2; ++s) acc = mfma32(dup[s], Tl[(tb + 2 * t + s) * 64 + l], acc);
+          if (L < 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = ((dmask >> (4 * t + r)) & 1u) ? acc[r] : 0.0f;
+          }
+          db[(L == 3 ? 12 : 4 * L) + t] += acc[0] + acc[1] + acc[2] + acc[3];
+          dn[t] = pack4v(acc);
+        } else {
+          dn[t] = bf16x4{0, 0, 0, 0};
+        }
+      }
+      const int base_t = L == 3 ? 0 : (L == 2 ? 12 : (L == 1 ? 28 : 44));
+      const int nc = L == 0 ? 3 : 4;
+#pragma unroll
+      for (int mo = 0; mo < 4; ++mo)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          if (mo < nt && ni < nc)
+            dw[base_t + nc * mo + ni] =
+                __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(dn[mo], hn[ni], dw[base_t + nc * mo + ni], 0, 0, 0);
+      // δ_L sample-major, for the next iteration: (W_{L+1}ᵀ δ_{L+1}) ⊙ ReLU'
+      if (L == 2 || L == 1) {
+        const uint32_t sm = L == 2 ? m3 : m2;
+        f32x4 acc[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int s = 0; s < 2; ++s) acc[mt] = mfma32(Tl[(tb + 2 * mt + s) * 64 + l], dup[s], acc[mt]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[mt][r] = ((sm >> (4 * mt + r)) & 1u) ? acc[mt][r] : 0.0f;
+        }
+        dup[0] = pack8(acc[0], acc[1]);
+        dup[1] = pack8(acc[2], acc[3]);
+      }
+      dmask = nm;
+    }
+  }
+  // ---- block reduction: waves 1..3 hand their tiles to wave 0, 14 at a time
+#pragma unroll
+  for (int chunk = 0; chunk < 4; ++chunk) {
+    if (wave > 0) {
+#pragma unroll
+      for (int t = 0; t < 14; ++t) xch[((wave - 1) * 14 + t) * 64 + l] = dw[14 * chunk + t];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int t = 0; t < 14; ++t)
+#pragma unroll
+        for (int w = 0; w < 3; ++w) dw[14 * chunk + t] += xch[(w * 14 + t) * 64 + l];
     }
     __syncthreads();
   }
-  // ---- partial gradient of this block
-  float *P = partial + static_cast<size_t>(blockIdx.x) * G_PARTIAL;
+  // biases, mix logit, loss: over the lane groups, then the waves
 #pragma unroll
-  for (int t = 0; t < 14; ++t) {
-    const int j = 14 * wave + t;
-    int L, mo, ni;
-    if (j < 12) L = 0, mo = j / 3, ni = j % 3;
-    else if (j < 28) L = 1, mo = (j - 12) / 4, ni = (j - 12) % 4;
-    else if (j < 44) L = 2, mo = (j - 28) / 4, ni = (j - 28) % 4;
-    else L = 3, mo = (j - 44) / 4, ni = (j - 44) % 4;
-    int off, rows_, cols, mts, kts, tb;
-    layer_geom(L, &off, &rows_, &cols, &mts, &kts, &tb);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) P[off + (16 * mo + 4 * fg + r) * cols + 16 * ni + sl] = dw[t][r];
+  for (int k = 0; k < 15; ++k) {
+    float v = db[k];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    if (g == 0) {
+      const int idx = k < 12 ? 64 * (k / 4) + 16 * (k % 4) + j : 192 + 16 * (k - 12) + j;  // b4: lobe-order row
+      atomicAdd(&sums[idx], v);
+    }
   }
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 6; ++k) {
     dalpha += __shfl_xor(dalpha, 1 << k);
     loss += __shfl_xor(loss, 1 << k);
   }
   if (l == 0) {
-    atomicAdd(&scal[0], dalpha);
-    atomicAdd(&scal[1], loss);
+    atomicAdd(&sums[G_BIAS], dalpha);
+    atomicAdd(&sums[G_BIAS + 1], loss);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) P[G_B1 + i] = db[i];
+  float *P = partial + static_cast<size_t>(blockIdx.x) * G_PARTIAL;
+  if (wave == 0) {
+    // dW tile (L, mo, ni): lane (g, j) register r = dW[16 mo + 4 g + r][column 16 ni + j]
+#pragma unroll
+    for (int t = 0; t < 56; ++t) {
+      const int L = t < 12 ? 3 : (t < 28 ? 2 : (t < 44 ? 1 : 0));
+      const int k = t < 12 ? t : (t < 28 ? t - 12 : (t < 44 ? t - 28 : t - 44));
+      const int nc = L == 0 ? 3 : 4;
+      const int mo = k / nc, ni = k % nc;
+      int off, rws, cols;
+      layer_geom(L, &off, &rws, &cols);
+      const int col = L == 0 ? xslot_feature(ni, j) : 16 * ni + j;  // identity column 16 g + t = input slot (g, t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = 16 * mo + 4 * g + r;
+        const int row_ = L == 3 ? out_logical(rr) : rr;
+        if (row_ >= 0 && col >= 0) P[off + row_ * cols + col] = dw[t][r];
+      }
+    }
+  }
+  for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) {
+    if (i < 192) P[G_B1 + i] = sums[i];
+    else if (out_logical(i - 192) >= 0) P[G_B4 + out_logical(i - 192)] = sums[i];
+  }
   if (threadIdx.x == 0) {
-    P[G_NPARAM_PAD] = scal[0];
-    P[G_NPARAM_PAD + 1] = scal[1];
+    P[G_NPARAM_PAD] = sums[G_BIAS];
+    P[G_NPARAM_PAD + 1] = sums[G_BIAS + 1];
   }
 }
 
-// Sums block partials; mode 0 writes the gradient to grad, mode 1 applies
-// Adam (step `t`, learning rate lr) to the master parameters.
-__global__ void k_guide_reduce(const float *__restrict__ partial, int blocks, float *__restrict__ params,
-                               float *__restrict__ adam_m, float *__restrict__ adam_v, float *__restrict__ grad,
-                               float *__restrict__ loss_out, int mode, float lr, int step) {
+// acc[p] += the sum of the block partials of this y-chunk (PART_CHUNK of them).
+constexpr int PART_CHUNK = 16;
+__global__ void k_guide_reduce(const float *__restrict__ partial, int blocks, float *__restrict__ acc) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > G_NPARAM_PAD + 1) return;
-  float g = 0.0f;
-  for (int b = 0; b < blocks; ++b) g += partial[static_cast<size_t>(b) * G_PARTIAL + p];
+  const int b0 = blockIdx.y * PART_CHUNK, b1 = min(blocks, b0 + PART_CHUNK);
+  float s = 0.0f;
+  for (int b = b0; b < b1; ++b) s += partial[static_cast<size_t>(b) * G_PARTIAL + p];
+  if (b1 > b0) atomicAdd(&acc[p], s);
+}
+
+// Consumes acc (and clears it for the next step): mode 0 writes the gradient
+// to grad, mode 1 applies Adam (step `step`, learning rate lr).
+__global__ void k_guide_adam(float *__restrict__ acc, float *__restrict__ params, float *__restrict__ adam_m,
+                             float *__restrict__ adam_v, float *__restrict__ grad, float *__restrict__ loss_out,
+                             int mode, float lr, int step) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > G_NPARAM_PAD + 1) return;
+  float g = acc[p];
+  acc[p] = 0.0f;
   if (p == G_NPARAM_PAD + 1) {
     if (loss_out) *loss_out = g;
     return;

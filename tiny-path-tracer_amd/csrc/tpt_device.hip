@@ -201,9 +201,9 @@ struct tpt_guide {
   int n_cu = 256;
   float *params = nullptr;   // G_NPARAM_PAD + 1 (mix logit)
   float *adam_m = nullptr, *adam_v = nullptr;
-  uint32_t *tiles = nullptr;   // G_TILES x 128 words
-  uint32_t *ttiles = nullptr;  // G_TT_TILES x 128 words
   uint32_t *xtiles = nullptr;  // inference tiles, GX_TILES x 256 words
+  uint32_t *rtiles = nullptr;  // training tiles, GR_TILES x 256 words
+  float *gacc = nullptr;       // gradient accumulator (G_NPARAM_PAD + 2), cleared by k_guide_adam
   float *partial = nullptr;    // train blocks x G_PARTIAL
   float *scratch = nullptr;    // gradient / loss staging (G_NPARAM_PAD + 2)
   int train_blocks = 0;
@@ -217,8 +217,7 @@ namespace {
 
 int guide_pack(tpt_guide *g, hipStream_t stream) {
   hipLaunchKernelGGL(k_guide_pack, dim3((G_B1 + 255) / 256), dim3(256), 0, stream, g->params,
-                     reinterpret_cast<uint16_t *>(g->tiles), reinterpret_cast<uint16_t *>(g->ttiles),
-                     reinterpret_cast<uint16_t *>(g->xtiles));
+                     reinterpret_cast<uint16_t *>(g->xtiles), reinterpret_cast<uint16_t *>(g->rtiles));
   HIP_TRY(hipGetLastError());
   return TPT_OK;
 }
@@ -229,12 +228,14 @@ int guide_step(tpt_guide *g, const float *rows, uint64_t n, uint64_t row0, uint6
                float *grad, float *loss_dev, hipStream_t stream) {
   const int blocks = static_cast<int>(std::min<uint64_t>((n + 63) / 64, static_cast<uint64_t>(g->train_blocks)));
   const size_t lds = k_guide_train_lds();
-  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), lds, stream, g->tiles, g->ttiles, g->params, rows, n,
-                     row0, n_total, g->partial);
+  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), lds, stream, g->rtiles, g->params, rows, n, row0,
+                     n_total, g->partial);
   HIP_TRY(hipGetLastError());
   if (mode == 1) ++g->step;
-  hipLaunchKernelGGL(k_guide_reduce, dim3((G_NPARAM_PAD + 2 + 255) / 256), dim3(256), 0, stream, g->partial, blocks,
-                     g->params, g->adam_m, g->adam_v, grad, loss_dev, mode, lr, g->step);
+  const dim3 rgrid((G_NPARAM_PAD + 2 + 255) / 256, (blocks + PART_CHUNK - 1) / PART_CHUNK);
+  hipLaunchKernelGGL(k_guide_reduce, rgrid, dim3(256), 0, stream, g->partial, blocks, g->gacc);
+  hipLaunchKernelGGL(k_guide_adam, dim3((G_NPARAM_PAD + 2 + 255) / 256), dim3(256), 0, stream, g->gacc, g->params,
+                     g->adam_m, g->adam_v, grad, loss_dev, mode, lr, g->step);
   HIP_TRY(hipGetLastError());
   if (mode == 1) return guide_pack(g, stream);
   return TPT_OK;
@@ -572,8 +573,8 @@ int tpt_guide_create(uint64_t seed, tpt_guide **out) {
   std::mt19937_64 rng(seed);
   std::normal_distribution<float> nd(0.0f, 1.0f);
   for (int L = 0; L < 4; ++L) {
-    int off, rows, cols, mts, kts, base;
-    layer_geom(L, &off, &rows, &cols, &mts, &kts, &base);
+    int off, rows, cols;
+    layer_geom(L, &off, &rows, &cols);
     const float fan_in = L == 0 ? static_cast<float>(G_IN) : 64.0f;
     const float sd = std::sqrt(2.0f / fan_in) * (L == 3 ? 0.1f : 1.0f);
     for (int o = 0; o < rows; ++o)
@@ -592,10 +593,12 @@ int tpt_guide_create(uint64_t seed, tpt_guide **out) {
   HIP_TRY(hipMalloc(&g->params, np * sizeof(float)));
   HIP_TRY(hipMalloc(&g->adam_m, np * sizeof(float)));
   HIP_TRY(hipMalloc(&g->adam_v, np * sizeof(float)));
-  HIP_TRY(hipMalloc(&g->tiles, G_TILES * G_TILE_WORDS * sizeof(uint32_t)));
-  HIP_TRY(hipMalloc(&g->ttiles, G_TT_TILES * G_TILE_WORDS * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&g->xtiles, GX_TILES * GX_TILE_WORDS * sizeof(uint32_t)));
   HIP_TRY(hipMemset(g->xtiles, 0, GX_TILES * GX_TILE_WORDS * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&g->rtiles, GR_TILES * 256 * sizeof(uint32_t)));
+  HIP_TRY(hipMemset(g->rtiles, 0, GR_TILES * 256 * sizeof(uint32_t)));
+  HIP_TRY(hipMalloc(&g->gacc, (G_NPARAM_PAD + 2) * sizeof(float)));
+  HIP_TRY(hipMemset(g->gacc, 0, (G_NPARAM_PAD + 2) * sizeof(float)));
   HIP_TRY(hipMalloc(&g->partial, static_cast<size_t>(g->train_blocks) * G_PARTIAL * sizeof(float)));
   HIP_TRY(hipMalloc(&g->scratch, (G_NPARAM_PAD + 2) * sizeof(float)));
   HIP_TRY(hipMemcpy(g->params, h.data(), np * sizeof(float), hipMemcpyHostToDevice));
@@ -616,9 +619,9 @@ int tpt_guide_destroy(tpt_guide *g) {
   (void)hipFree(g->params);
   (void)hipFree(g->adam_m);
   (void)hipFree(g->adam_v);
-  (void)hipFree(g->tiles);
-  (void)hipFree(g->ttiles);
   (void)hipFree(g->xtiles);
+  (void)hipFree(g->rtiles);
+  (void)hipFree(g->gacc);
   (void)hipFree(g->partial);
   (void)hipFree(g->scratch);
   (void)hipFree(g->rec_side);
