@@ -128,7 +128,8 @@ typedef struct tpt_material {
 #define TPT_TEX_PERLIN 2u
 #define TPT_TEX_IMAGE 3u
 /* constant: p[0..2] colour.  checker: a = odd, b = even.  perlin: p[0] scale,
- * a = octaves.  image: a = byte offset into texels, b = width, c = height. */
+ * a = octaves, b = index of its Perlin tables (0 in a single scene).  image:
+ * a = byte offset into texels, b = width, c = height. */
 typedef struct tpt_texture {
   uint32_t kind;
   uint32_t a, b, c;
@@ -219,6 +220,7 @@ int tpt_host_scene_destroy(tpt_host_scene *hs);
 
 /* ------------------------------------------------------------- lifecycle --- */
 typedef struct tpt_scene tpt_scene;
+typedef struct tpt_guide tpt_guide; /* learned guiding proposal, below */
 
 int tpt_abi_version(void);
 const char *tpt_last_error(void);
@@ -286,6 +288,56 @@ typedef struct tpt_render_stats {
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
                float *d_sumsq, tpt_render_stats *stats, void *stream);
 
+/* ----------------------------------------------------- batched rendering ---
+ * No counterpart in the reference, which renders one scene per run: many
+ * renders (scenes, resolutions, sample counts) in one launch sequence.
+ *
+ * tpt_scene_create_pool packs several flattened scenes into one handle: their
+ * BVH slots, primitives, transforms, materials, textures and lights go into
+ * one pool with indices rebased.  tpt_trace_closest and tpt_render use scene
+ * 0 of a pool. */
+int tpt_scene_create_pool(const tpt_flat_scene *scenes, uint32_t n_scenes, tpt_scene **out);
+
+/* One render of a pooled scene (the fields of tpt_render_desc, its camera and
+ * its sums).  Samples [sample_offset, sample_offset + spp) of every pixel are
+ * added to d_sum / d_sumsq (3 * width * height floats each; d_sumsq may be
+ * NULL). */
+typedef struct tpt_job {
+  uint32_t scene;
+  uint32_t width, height, spp, max_depth;
+  uint32_t sample_offset;
+  uint64_t seed;
+  float background;
+  uint32_t reserved;
+  tpt_camera camera;
+  float *d_sum, *d_sumsq;
+} tpt_job;
+
+/* Launch settings of one tpt_render_jobs call. */
+typedef struct tpt_launch {
+  uint32_t queue_capacity;  /* path records in flight; 0 = 2^21 */
+  uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 8 */
+  uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
+                               even); 0 = one launch at a time */
+  uint32_t flags;           /* TPT_RENDER_PROFILE or 0 (profiling launches one at a time) */
+  uint32_t guiding;         /* with a guide: training-record stride (0 = none) */
+  uint32_t guide_vertices;  /* with a guide: guided diffuse vertices per path (0 = 2) */
+} tpt_launch;
+
+/* Renders every job through one shared work cursor (jobs of different scenes
+ * share every launch).  Each job's result equals tpt_render (guide NULL) or
+ * tpt_render_guided of that job alone, bit for bit. */
+int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *launch,
+                    tpt_guide *guide, tpt_render_stats *stats, void *stream);
+
+/* Autotuner: times calibration renders of the jobs (at most calib_spp
+ * samples each, 0 = 2, into scratch sums) over queue capacities (2^20..2^23),
+ * grid sizes (4, 8, 16 blocks per CU) and graph replay lengths (0, 8), and
+ * writes the fastest settings into *best (its flags, guiding and
+ * guide_vertices are kept).  *best_ms: that calibration's wall time. */
+int tpt_autotune(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *guide, uint32_t calib_spp,
+                 tpt_launch *best, float *best_ms);
+
 /* ------------------------------------------------------- learned guiding ---
  * No counterpart in the reference: an online-trained proposal, mixed into the
  * reference's mixture_pdf(hitable_pdf, cosine_pdf) by one-sample MIS so that
@@ -308,7 +360,6 @@ int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc
  * 2 unused. */
 #define TPT_GUIDE_NPARAM 14576
 #define TPT_GUIDE_ROW 16
-typedef struct tpt_guide tpt_guide;
 
 int tpt_guide_create(uint64_t seed, tpt_guide **out);
 int tpt_guide_destroy(tpt_guide *guide);

@@ -1,0 +1,151 @@
+"""Batched rendering (tpt_scene_create_pool + tpt_render_jobs) against single renders.
+
+Each job of a batch draws the same Philox streams as a tpt_render of that job
+alone and its samples are summed in the same order, so the sums must agree
+bit for bit, whatever the other jobs, the pass split, the queue capacity, the
+grid size or the captured-graph replay length.
+"""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["cornell555", "random_spheres", "ref_two_perlin_spheres", "random_spheres_textured", "box_list",
+         "ref_cornell_box"]
+
+
+@pytest.fixture(scope="module")
+def hosts(tpt):
+    return [tpt.HostScene(n) for n in NAMES]
+
+
+@pytest.fixture(scope="module")
+def pool(tpt, device, hosts):
+    return tpt.Scene.pool(hosts)
+
+
+@pytest.fixture(scope="module")
+def singles(tpt, device, hosts):
+    return [tpt.Scene(h) for h in hosts]
+
+
+def specs(hosts):
+    """One job per scene: the scene's camera at a fraction of its resolution."""
+    out = []
+    for i, h in enumerate(hosts):
+        k = [20, 6, 10, 25, 12, 14][i]
+        w, hh = max(8, h.info.width // k), max(8, h.info.height // k)
+        out.append(dict(scene=i, width=w, height=hh, spp=[3, 6, 4, 2, 5, 3][i], seed=11 + i, sample_offset=i,
+                        max_depth=[50, 50, 20, 50, 12, 50][i]))
+    return out
+
+
+def single_render(tpt, sc, host, sp, guide=None):
+    cam = host.camera
+    s, q, st = sc.render(width=sp["width"], height=sp["height"], spp=sp["spp"], seed=sp["seed"],
+                         sample_offset=sp["sample_offset"], max_depth=sp["max_depth"], camera=cam, guide=guide)
+    return s, q, st
+
+
+def batch_jobs(pool, sps):
+    jobs, bufs = [], []
+    for sp in sps:
+        j, s, q = pool.job(scene=sp["scene"], width=sp["width"], height=sp["height"], spp=sp["spp"], seed=sp["seed"],
+                           sample_offset=sp["sample_offset"], max_depth=sp["max_depth"])
+        jobs.append(j)
+        bufs.append((s, q))
+    return jobs, bufs
+
+
+@pytest.mark.parametrize("graph_iters,blocks", [(0, 8), (8, 8), (6, 4), (0, 2)])
+def test_jobs_equal_single_renders(tpt, pool, singles, hosts, graph_iters, blocks):
+    import torch
+
+    sps = specs(hosts)
+    jobs, bufs = batch_jobs(pool, sps)
+    L = tpt.Launch()
+    L.graph_iters = graph_iters
+    L.blocks_per_cu = blocks
+    L.queue_capacity = 1 << 18
+    st = pool.render_jobs(jobs, launch=L)
+    assert st["paths"] == sum(sp["width"] * sp["height"] * sp["spp"] for sp in sps)
+    for sp, (s, q) in zip(sps, bufs):
+        s1, q1, _ = single_render(tpt, singles[sp["scene"]], hosts[sp["scene"]], sp)
+        print(NAMES[sp["scene"]], sp["width"], sp["height"], float(s.mean()), float(s1.mean()))
+        assert torch.equal(s, s1), NAMES[sp["scene"]]
+        assert torch.equal(q, q1), NAMES[sp["scene"]]
+
+
+def test_one_job_of_a_pooled_scene(tpt, pool, singles, hosts):
+    import torch
+
+    sp = specs(hosts)[2]  # a Perlin scene: its tables are found through the pooled texture
+    jobs, bufs = batch_jobs(pool, [sp])
+    pool.render_jobs(jobs)
+    s1, q1, _ = single_render(tpt, singles[2], hosts[2], sp)
+    assert torch.equal(bufs[0][0], s1) and torch.equal(bufs[0][1], q1)
+
+
+def test_jobs_spanning_several_passes(tpt, pool, singles, hosts):
+    """2^26 items per pass: 1024x1024 at 70 spp needs two passes, split mid-job."""
+    import torch
+
+    sps = [dict(scene=0, width=1024, height=1024, spp=70, seed=3, sample_offset=0, max_depth=50),
+           dict(scene=4, width=256, height=256, spp=9, seed=4, sample_offset=2, max_depth=50)]
+    jobs, bufs = batch_jobs(pool, sps)
+    L = tpt.Launch()
+    L.graph_iters = 8
+    pool.render_jobs(jobs, launch=L)
+    for sp, (s, q) in zip(sps, bufs):
+        s1, q1, _ = single_render(tpt, singles[sp["scene"]], hosts[sp["scene"]], sp)
+        assert torch.equal(s, s1) and torch.equal(q, q1)
+
+
+def test_guided_jobs_equal_single_guided_renders(tpt, pool, singles, hosts):
+    import torch
+
+    g = tpt.Guide(seed=5)
+    p = g.get_params()
+    p[-1] = 0.5  # mix weight sigmoid(0.5)
+    g.set_params(p)
+    sps = specs(hosts)
+    jobs, bufs = batch_jobs(pool, sps)
+    L = tpt.Launch()
+    L.graph_iters = 8
+    pool.render_jobs(jobs, launch=L, guide=g)
+    for sp, (s, q) in zip(sps, bufs):
+        s1, q1, _ = single_render(tpt, singles[sp["scene"]], hosts[sp["scene"]], sp, guide=g)
+        assert torch.equal(s, s1) and torch.equal(q, q1), NAMES[sp["scene"]]
+
+
+def test_autotune_picks_a_valid_launch(tpt, pool, hosts):
+    import torch
+
+    sps = specs(hosts)
+    jobs, _ = batch_jobs(pool, sps)
+    best, ms = pool.autotune(jobs, calib_spp=1)
+    print("autotune:", best.as_dict(), f"{ms:.2f} ms")
+    assert best.queue_capacity in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert best.blocks_per_cu in (4, 8, 16) and best.graph_iters in (0, 8) and ms > 0
+    # the tuned launch gives the same sums as the default one
+    jobs_a, bufs_a = batch_jobs(pool, sps)
+    jobs_b, bufs_b = batch_jobs(pool, sps)
+    pool.render_jobs(jobs_a, launch=best)
+    pool.render_jobs(jobs_b)
+    for (sa, qa), (sb, qb) in zip(bufs_a, bufs_b):
+        assert torch.equal(sa, sb) and torch.equal(qa, qb)
+
+
+def test_bad_jobs_are_rejected(tpt, pool, hosts):
+    jobs, _ = batch_jobs(pool, specs(hosts)[:1])
+    jobs[0].scene = len(NAMES)
+    with pytest.raises(tpt.TptError, match="scene out of range"):
+        pool.render_jobs(jobs)
+    jobs, _ = batch_jobs(pool, specs(hosts)[:1])
+    jobs[0].max_depth = 70000
+    with pytest.raises(tpt.TptError, match="max_depth"):
+        pool.render_jobs(jobs)
+    L = tpt.Launch()
+    L.guiding = 4
+    jobs, _ = batch_jobs(pool, specs(hosts)[:1])
+    with pytest.raises(tpt.TptError, match="guiding needs"):
+        pool.render_jobs(jobs, launch=L)

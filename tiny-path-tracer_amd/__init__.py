@@ -78,6 +78,23 @@ class RenderDesc(C.Structure):
                 ("guide_vertices", C.c_uint32)]
 
 
+class Job(C.Structure):
+    """tpt_job: one render of a pooled scene."""
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("sample_offset", C.c_uint32), ("seed", C.c_uint64),
+                ("background", C.c_float), ("reserved", C.c_uint32), ("camera", Camera), ("d_sum", C.c_void_p),
+                ("d_sumsq", C.c_void_p)]
+
+
+class Launch(C.Structure):
+    """tpt_launch: settings of one tpt_render_jobs call."""
+    _fields_ = [("queue_capacity", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("graph_iters", C.c_uint32),
+                ("flags", C.c_uint32), ("guiding", C.c_uint32), ("guide_vertices", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("segments", C.c_uint64), ("light_queries", C.c_uint64),
                 ("nan_samples", C.c_uint64), ("kernel_ms", C.c_double), ("nodes_visited", C.c_uint64),
@@ -116,6 +133,9 @@ def lib():
     L.tpt_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp, C.POINTER(RenderStats), vp]
     L.tpt_render_guided.argtypes = [vp, vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp,
                                     C.POINTER(RenderStats), vp]
+    L.tpt_scene_create_pool.argtypes = [C.POINTER(FlatScene), u32, C.POINTER(vp)]
+    L.tpt_render_jobs.argtypes = [vp, C.POINTER(Job), u32, C.POINTER(Launch), vp, C.POINTER(RenderStats), vp]
+    L.tpt_autotune.argtypes = [vp, C.POINTER(Job), u32, vp, u32, C.POINTER(Launch), C.POINTER(C.c_float)]
     L.tpt_guide_create.argtypes = [u64, C.POINTER(vp)]
     L.tpt_guide_destroy.argtypes = [vp]
     L.tpt_guide_get_params.argtypes = [vp, vp]
@@ -172,20 +192,74 @@ def _torch():
 
 
 class Scene:
-    """A scene uploaded to the current HIP device."""
+    """A scene (or a pool of scenes) uploaded to the current HIP device."""
 
-    def __init__(self, host_scene):
+    def __init__(self, host_scene, pool=None):
         L = lib()
         if L.tpt_device_count() < 1:
             raise TptError("no gfx950 device visible: the device path has no CPU fallback")
-        self.host = host_scene
+        self.hosts = list(pool) if pool is not None else [host_scene]
+        self.host = self.hosts[0]
         h = C.c_void_p()
-        _check(L.tpt_scene_create(C.byref(host_scene.flat), C.byref(h)), "tpt_scene_create")
+        if pool is None:
+            _check(L.tpt_scene_create(C.byref(host_scene.flat), C.byref(h)), "tpt_scene_create")
+        else:
+            arr = (FlatScene * len(self.hosts))(*[hs.flat for hs in self.hosts])
+            _check(L.tpt_scene_create_pool(arr, len(self.hosts), C.byref(h)), "tpt_scene_create_pool")
         self.h = h
 
     @classmethod
     def named(cls, name, seed=5489):
         return cls(HostScene(name, seed))
+
+    @classmethod
+    def pool(cls, host_scenes):
+        """Several HostScenes in one handle (tpt_scene_create_pool)."""
+        return cls(None, pool=host_scenes)
+
+    def job(self, scene=0, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0,
+            background=None, camera=None, sum_buf=None, sumsq_buf=None):
+        """A Job for render_jobs, with zeroed device sums unless given; returns
+        (job, sum, sumsq)."""
+        torch = _torch()
+        info = self.hosts[scene].info
+        j = Job()
+        j.scene = scene
+        j.width = width or info.width
+        j.height = height or info.height
+        j.spp = info.spp if spp is None else spp
+        j.max_depth = info.max_depth if max_depth is None else max_depth
+        j.seed = seed
+        j.sample_offset = sample_offset
+        j.background = info.background if background is None else background
+        j.camera = camera if camera is not None else self.hosts[scene].camera
+        shape = (j.height, j.width, 3)
+        if sum_buf is None:
+            sum_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        if sumsq_buf is None:
+            sumsq_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        j.d_sum = sum_buf.data_ptr()
+        j.d_sumsq = sumsq_buf.data_ptr() if sumsq_buf is not False else None
+        return j, sum_buf, sumsq_buf
+
+    def render_jobs(self, jobs, launch=None, guide=None, stream=None):
+        """tpt_render_jobs over a list of Job; returns the stats dict."""
+        torch = _torch()
+        arr = (Job * len(jobs))(*jobs)
+        stats = RenderStats()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(lib().tpt_render_jobs(self.h, arr, len(jobs), C.byref(launch) if launch is not None else None,
+                                     guide.h if guide is not None else None, C.byref(stats), s), "tpt_render_jobs")
+        return stats.as_dict()
+
+    def autotune(self, jobs, guide=None, calib_spp=2, base=None):
+        """tpt_autotune: returns (Launch, calibration ms)."""
+        arr = (Job * len(jobs))(*jobs)
+        best = base if base is not None else Launch()
+        ms = C.c_float()
+        _check(lib().tpt_autotune(self.h, arr, len(jobs), guide.h if guide is not None else None, calib_spp,
+                                  C.byref(best), C.byref(ms)), "tpt_autotune")
+        return best, ms.value
 
     def close(self):
         if getattr(self, "h", None):

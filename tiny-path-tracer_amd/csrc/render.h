@@ -1,22 +1,29 @@
 // Wavefront renderer kernels (replace color() + the sample loop of the
 // reference: src/utils.cc:58-94, main.cpp:115-142).
 //
-// Work: one render pass covers samples [s0, s1) of every pixel; item w of the
-// pass is (pixel w % n_pix, sample s0 + w / n_pix).  A fixed number of path
-// records is in flight; a finished path writes its de_nan'ed sample to
-// res[w] and claims the next item from a global cursor, so the GPU stays full
-// until the pass runs out of work.  At the end of a pass k_accumulate adds
-// res to the caller's per-pixel sums in sample order, so the result does not
-// depend on scheduling.
+// Work: a render call covers one or more jobs (an image of one scene of the
+// scene pool: camera, resolution, samples, seed).  Its work items are the
+// jobs' (sample, pixel) pairs, job after job, sample-major within a job.  A
+// pass covers a run of whole sample layers of at most 2^26 items; the pass
+// table maps pass item w to its job, pixel and sample.  A fixed number of
+// path records is in flight; a finished path writes its de_nan'ed sample to
+// res[w] and claims the next item from a shared cursor, so the GPU stays full
+// until the pass runs out of work whatever the mix of jobs.  At the end of a
+// pass k_accumulate adds res to each job's per-pixel sums in sample order, so
+// the result does not depend on scheduling or on how jobs share a pass.
 //
 // Queues hold path records by value, as planes of float4 (one 16-byte plane
 // per field group), so every queue access is a coalesced dwordx4:
-//   ray queue (ping-pong), 4 planes: (o, item) (d, depth) (throughput, -)
-//                                    (radiance, -)
-//   material queues (diffuse / specular / terminal), the 4 ray planes plus
+//   ray queue (ping-pong), 3 planes: (o, item) (d, depth | pass entry)
+//                                    (throughput, diffuse count | NaN bits)
+//   material queues (diffuse / other / terminal), the 3 ray planes plus
 //                                    (t, slot, entry, material)
 // Appends use a 64-lane ballot + popcount prefix and one atomic per wave, on
 // one of 64 sub-queue counters (see NSUB).
+//
+// MULTI (template flag): the launch spans several jobs, so each lane looks up
+// its own job and scene; without it every lane belongs to job 0 and those
+// lookups are wave-uniform.
 #ifndef TPT_CSRC_RENDER_H
 #define TPT_CSRC_RENDER_H
 
@@ -56,20 +63,47 @@ struct render_state {
   unsigned long long *stats;  // ST_COUNT per block
 };
 
-struct render_params {
+// One image of one scene (device copy of a tpt_job).
+struct job_dev {
   tpt_camera cam;
-  uint32_t width, height, n_pix;
-  uint32_t s0, pass_spp;  // pass: samples [s0, s0 + pass_spp)
-  uint32_t n_items;       // n_pix * pass_spp
-  uint32_t n_first;       // items [0, n_first) seed the first ray queue
-  uint32_t max_depth;
+  uint32_t width, height, n_pix, max_depth;
   uint64_t seed;
   float background;
+  uint32_t scene;
   float *sum, *sumsq;
+};
+
+// A scene of the pool: its segments and lights in the pooled arrays, and its
+// box (position encoding of the guide).
+struct scene_dev {
+  uint32_t seg_begin, n_seg, light_begin, n_lights, light_mode, pad[3];
+  float bounds[6], pad2[2];
+};
+
+// Pass table: items [item_begin, next entry's item_begin) of the pass are
+// samples [s_begin, s_begin + n_layers) of job `job`, sample-major;
+// acc_begin is the prefix of the jobs' pixel counts (k_accumulate's index).
+struct pass_ent {
+  uint32_t item_begin, job, s_begin, n_layers, acc_begin, pad[3];
+};
+struct pass_hdr {
+  uint32_t n_items;  // items of the pass
+  uint32_t n_first;  // items [0, n_first) seed the first ray queue
+  uint32_t n_ent;    // pass table entries
+  uint32_t n_acc;    // sum of the entries' pixel counts
+};
+
+// Kernel parameters: constant over a render call (per-pass values live in the
+// device-side pass header and table, so a captured bounce loop can be
+// replayed in every pass).
+struct render_params {
+  const job_dev *jobs;
+  const scene_dev *scenes;
+  const pass_hdr *hdr;
+  const pass_ent *pass;
   // guided mode
   const uint32_t *g_tiles;  // bf16 inference weight tiles (16x16x32 layout)
   const float *g_params;    // fp32 parameters (biases, mix logit)
-  float bounds[6];          // scene box for the position encoding
   uint32_t guide_verts;     // diffuse vertices per path that use the guide (~0u: every one, plain mode)
   uint32_t rec_stride;      // record training rows from items with item % rec_stride == 0 (0: off)
   float4 *rec_side;         // per recorded path: 2 vertices x 5 float4
@@ -81,8 +115,47 @@ constexpr uint32_t REC_VERTS = 2;
 struct path {
   v3 o, d, thr, rad;
   uint32_t item, depth;
+  uint32_t ent;    // pass table entry of the item (its job)
   uint32_t ndiff;  // diffuse (lambertian) vertices so far
 };
+
+// Where a pass item comes from.
+struct item_ctx {
+  uint32_t ent, job, px, smp;
+};
+
+template <bool MULTI>
+__device__ __forceinline__ uint32_t find_entry(const render_params &P, uint32_t n_ent, uint32_t item) {
+  if (!MULTI) return 0;
+  uint32_t lo = 0, hi = n_ent;  // invariant: pass[lo].item_begin <= item < pass[hi].item_begin
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (P.pass[mid].item_begin <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ item_ctx item_in_entry(const render_params &P, uint32_t ent, uint32_t item) {
+  const pass_ent e = P.pass[ent];
+  const uint32_t n_pix = P.jobs[e.job].n_pix;
+  const uint32_t local = item - e.item_begin;
+  return item_ctx{ent, e.job, local % n_pix, e.s_begin + local / n_pix};
+}
+
+// The scene view of a job: the pooled arrays with that scene's segment and
+// light ranges (without MULTI the caller's view already is job 0's scene).
+template <bool MULTI>
+__device__ __forceinline__ scene_view view_of(const scene_view &S, const render_params &P, uint32_t job) {
+  if (!MULTI) return S;
+  const scene_dev &sc = P.scenes[P.jobs[job].scene];
+  scene_view v = S;
+  v.segments = S.segments + 2 * sc.seg_begin;
+  v.n_segments = sc.n_seg;
+  v.lights = S.lights + sc.light_begin;
+  v.n_lights = sc.n_lights;
+  v.light_mode = sc.light_mode;
+  return v;
+}
 
 __device__ __forceinline__ uint32_t lane_prefix(unsigned long long mask) {
   return __popcll(mask & ((1ull << __lane_id()) - 1ull));
@@ -102,8 +175,8 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
 }
 
 // Items [n_first, n_items) are split into NSUB contiguous shards, one cursor each.
-__device__ __forceinline__ uint32_t shard_lo(const render_params &P, uint32_t k) {
-  return P.n_first + static_cast<uint32_t>((static_cast<uint64_t>(P.n_items - P.n_first) * k) / NSUB);
+__device__ __forceinline__ uint32_t shard_lo(const pass_hdr &H, uint32_t k) {
+  return H.n_first + static_cast<uint32_t>((static_cast<uint64_t>(H.n_items - H.n_first) * k) / NSUB);
 }
 
 // Exclusive prefix over n (<= 4 * 64) region counts into pre[0..n]; one wave
@@ -149,12 +222,12 @@ __device__ __forceinline__ uint32_t find_region(const uint32_t *pre, uint32_t n,
 // Lanes with `want` receive the next unclaimed work item (or >= n_items when
 // the pass has none left).  One atomic per wave per shard tried; a wave starts
 // at its own shard and moves on when that one runs dry.  All lanes call.
-__device__ __forceinline__ uint32_t claim_item(const render_params &P, uint32_t *counters, bool want) {
+__device__ __forceinline__ uint32_t claim_item(const pass_hdr &H, uint32_t *counters, bool want) {
   unsigned long long need = __ballot(want);
   uint32_t item = 0xFFFFFFFFu;
   uint32_t k = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) % NSUB;
   for (uint32_t attempt = 0; attempt < NSUB && need; ++attempt, k = (k + 1) % NSUB) {
-    const uint32_t hi = shard_lo(P, k + 1);
+    const uint32_t hi = shard_lo(H, k + 1);
     uint32_t *cur = counters + cur_at(k);
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= hi)
       continue;
@@ -186,7 +259,8 @@ __device__ __forceinline__ void block_stats(unsigned long long *stats, const uin
   if (threadIdx.x < static_cast<uint32_t>(n)) stats[blockIdx.x * ST_COUNT + threadIdx.x] += acc[threadIdx.x];
 }
 
-// Path record, 48 bytes in 3 planes.  A live path's radiance is 0 or NaN per
+// Path record, 48 bytes in 3 planes (depth and pass entry share a word: both
+// are below 2^16).  A live path's radiance is 0 or NaN per
 // channel: light is only added by the event that ends the path (an emitter or
 // a miss), and the reference's `emitted + ...` form turns a channel NaN when a
 // non-finite throughput meets a zero term.  So only its NaN bits are stored
@@ -195,7 +269,7 @@ __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const p
   const uint32_t nan_bits = (p.rad.x != p.rad.x ? 1u : 0u) | (p.rad.y != p.rad.y ? 2u : 0u) |
                             (p.rad.z != p.rad.z ? 4u : 0u);
   q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
-  q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth));
+  q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth | (p.ent << 16)));
   q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16)));
 }
 
@@ -205,7 +279,8 @@ __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
   p.o = v3{a.x, a.y, a.z};
   p.item = __float_as_uint(a.w);
   p.d = v3{b.x, b.y, b.z};
-  p.depth = __float_as_uint(b.w);
+  p.depth = __float_as_uint(b.w) & 0xFFFFu;
+  p.ent = __float_as_uint(b.w) >> 16;
   p.thr = v3{c.x, c.y, c.z};
   const uint32_t w = __float_as_uint(c.w);
   p.ndiff = w & 0xFFFFu;
@@ -216,20 +291,19 @@ __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
 
 __device__ __forceinline__ v3 cam_vec(const float *a) { return v3{a[0], a[1], a[2]}; }
 
-// Camera path for work item `item`: jitter (main.cpp:121-122, in double as
-// there), then camera_with_blur::get_ray (src/camera.cc:23-31).  Bounce 0 of
-// the (pixel, sample) stream.
-__device__ __forceinline__ path raygen(const render_params &P, uint32_t item) {
-  const uint32_t px = item % P.n_pix;
-  const uint32_t s = P.s0 + item / P.n_pix;
+// Camera path for work item `item` of job ctx.job: jitter (main.cpp:121-122,
+// in double as there), then camera_with_blur::get_ray (src/camera.cc:23-31).
+// Bounce 0 of the (pixel, sample) stream.
+__device__ __forceinline__ path raygen(const render_params &P, uint32_t item, const item_ctx &c) {
+  const job_dev &J = P.jobs[c.job];
   stream R;
-  R.init(px, s, 0, P.seed);
-  const uint32_t i = px % P.width, j = px / P.width;
+  R.init(c.px, c.smp, 0, J.seed);
+  const uint32_t i = c.px % J.width, j = c.px / J.width;
   const float u = static_cast<float>((static_cast<double>(static_cast<float>(i)) + static_cast<double>(R.next())) /
-                                     static_cast<double>(static_cast<float>(P.width)));
+                                     static_cast<double>(static_cast<float>(J.width)));
   const float v = static_cast<float>((static_cast<double>(static_cast<float>(j)) + static_cast<double>(R.next())) /
-                                     static_cast<double>(static_cast<float>(P.height)));
-  const tpt_camera &C = P.cam;
+                                     static_cast<double>(static_cast<float>(J.height)));
+  const tpt_camera &C = J.cam;
   const v3 disk = in_unit_disk(R);
   const v3 rd = disk * C.lens_radius;
   const v3 off = cam_vec(C.u) * rd.x + cam_vec(C.v) * rd.y;
@@ -242,8 +316,14 @@ __device__ __forceinline__ path raygen(const render_params &P, uint32_t item) {
   p.rad = v3{0.0f, 0.0f, 0.0f};
   p.item = item;
   p.depth = 0;
+  p.ent = c.ent;
   p.ndiff = 0;
   return p;
+}
+
+template <bool MULTI>
+__device__ __forceinline__ path raygen_item(const render_params &P, const pass_hdr &H, uint32_t item) {
+  return raygen(P, item, item_in_entry(P, find_entry<MULTI>(P, H.n_ent, item), item));
 }
 
 __global__ void k_zero_stats(unsigned long long *stats, uint32_t n) {
@@ -252,27 +332,30 @@ __global__ void k_zero_stats(unsigned long long *stats, uint32_t n) {
 
 // First items of a pass, straight into ray queue 0 (item w -> region w % NSUB,
 // entry w / NSUB; no atomics); resets every counter of the pass.
+template <bool MULTI>
 __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
+  const pass_hdr H = *P.hdr;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t w = gid; w < COUNTER_WORDS / CSTRIDE; w += stride) {
     const uint32_t set = w / NSUB, k = w % NSUB;
     uint32_t v = 0;
-    if (set == 0) v = (P.n_first + NSUB - 1 - k) / NSUB;  // ray queue 0 region counts
-    if (set == 8) v = shard_lo(P, k);                       // shard cursors
+    if (set == 0) v = (H.n_first + NSUB - 1 - k) / NSUB;  // ray queue 0 region counts
+    if (set == 8) v = shard_lo(H, k);                       // shard cursors
     Q.counters[w * CSTRIDE] = v;
   }
-  for (uint32_t w = gid; w < P.n_first; w += stride)
-    store_ray(Q.rq[0], (w % NSUB) * Q.region + w / NSUB, raygen(P, w));
+  for (uint32_t w = gid; w < H.n_first; w += stride)
+    store_ray(Q.rq[0], (w % NSUB) * Q.region + w / NSUB, raygen_item<MULTI>(P, H, w));
 }
 
 // SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  The
 // slot/entry arrays come in as separate restrict arguments so that
 // wave-uniform loads from them can use the scalar data path.
-template <bool SWEEP>
+template <bool SWEEP, bool MULTI>
 __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
+  static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");
   S.slots = slots;
   S.entries = entries;
   __shared__ uint32_t offs[NSUB];
@@ -297,13 +380,15 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
       const uint32_t r = find_region(pre, NSUB, qi);
       p = load_ray(in, r * Q.region + (qi - pre[r]));
     }
-    const bool got = trace_world<SWEEP>(S, p.o, p.d, 0.001f, kFltMax, &h, &st[ST_NODES], valid);
+    const uint32_t job = MULTI ? P.pass[p.ent].job : 0u;
+    const scene_view Sj = view_of<MULTI>(S, P, job);
+    const bool got = trace_world<SWEEP>(Sj, p.o, p.d, 0.001f, kFltMax, &h, &st[ST_NODES], valid);
     if (valid) {
       ++st[ST_SEGMENTS];
       if (got) {
         h.mat &= TPT_PAYLOAD_MASK;
         const uint32_t kind = S.materials[h.mat].kind;
-        if (kind == TPT_MAT_DIFFUSE_LIGHT || p.depth >= P.max_depth) {
+        if (kind == TPT_MAT_DIFFUSE_LIGHT || p.depth >= P.jobs[job].max_depth) {
           cls = Q_TERMINAL;
         } else {
           // In guided mode only the first guide_verts diffuse vertices of a
@@ -381,12 +466,13 @@ __device__ __forceinline__ void finish_sample(const render_params &P, const rend
 
 // Finished lanes claim the next work item; every continuing path goes to the
 // next ray queue.  All lanes call.
-__device__ __forceinline__ void claim_and_append(const render_params &P, const render_state &Q, uint32_t *next_count,
-                                                 float4 *const *out, uint32_t home, path &p, bool cont,
-                                                 bool finished) {
-  const uint32_t item = claim_item(P, Q.counters, finished);
-  if (finished && item < P.n_items) {
-    p = raygen(P, item);
+template <bool MULTI>
+__device__ __forceinline__ void claim_and_append(const render_params &P, const pass_hdr &H, const render_state &Q,
+                                                 uint32_t *next_count, float4 *const *out, uint32_t home, path &p,
+                                                 bool cont, bool finished) {
+  const uint32_t item = claim_item(H, Q.counters, finished);
+  if (finished && item < H.n_items) {
+    p = raygen_item<MULTI>(P, H, item);
     cont = true;
   }
   const uint32_t at = home * Q.region + wave_append(next_count, cont);
@@ -396,8 +482,9 @@ __device__ __forceinline__ void claim_and_append(const render_params &P, const r
 // Shading of the material queues.  Plain mode: all three classes.  Guided
 // mode: the other and terminal classes; the diffuse class (the vertices that
 // use the guide) is shaded by k_guide_shade.
-template <bool GUIDED>
+template <bool GUIDED, bool MULTI>
 __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+  const pass_hdr H = *P.hdr;
   // material regions in class-major order: region r = (class - C0) * NSUB + k
   constexpr uint32_t C0 = GUIDED ? 1u : 0u, NR = (3u - C0) * NSUB;
   __shared__ uint32_t offs[NR];
@@ -425,18 +512,21 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
       p = load_ray(Q.mq[cls], off);
       const float4 hw = Q.mq[cls][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
+      const job_dev &J = P.jobs[c.job];
+      const scene_view Sj = view_of<MULTI>(S, P, c.job);
       if (h.slot == kNone) {
         // miss: black (src/utils.cc:85-86) or the sky extension
-        const v3 bg = P.background > 0.0f ? sky(p.d, P.background) : v3{0.0f, 0.0f, 0.0f};
+        const float bgs = J.background;
+        const v3 bg = bgs > 0.0f ? sky(p.d, bgs) : v3{0.0f, 0.0f, 0.0f};
         p.rad = p.rad + p.thr * bg;
       } else {
         v3 pt, n;
         float u, v;
         hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
         const tpt_material M = S.materials[h.mat];
-        const uint32_t px = p.item % P.n_pix, smp = P.s0 + p.item / P.n_pix;
         stream R;
-        R.init(px, smp, p.depth + 1, P.seed);
+        R.init(c.px, c.smp, p.depth + 1, J.seed);
         v3 nd{0.0f, 0.0f, 0.0f};
         if (cls == Q_TERMINAL) {
           // diffuse_light::emitted is one-sided (src/material.cc:79-86);
@@ -481,13 +571,13 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           const v3 atten = tex_value(S, M.tex, u, v, pt);
           const onb uvw(n);
           float pdf;
-          if (S.n_lights > 0) {
+          if (Sj.n_lights > 0) {
             if (R.next() < 0.5f) {
-              nd = light_random(S, pt, R);
+              nd = light_random(Sj, pt, R);
             } else {
               nd = uvw.local(on_hemisphere(R));
             }
-            const float pl = light_pdf(S, pt, nd);
+            const float pl = light_pdf(Sj, pt, nd);
             ++st[ST_LIGHT];
             float pc = dot(unit(nd), uvw.w);
             pc = pc > 0.0f ? pc : 0.0f;
@@ -516,7 +606,7 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
         finished = true;
       }
     }
-    claim_and_append(P, Q, next_count, out, home, p, cont, finished);
+    claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
   }
   block_stats(Q.stats, st, ST_COUNT);
 }
@@ -534,7 +624,9 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
 // weighted atten (cos/π) q / (p_ref m), p_ref being the reference's mixture
 // value (cosine_pdf::value is cos θ without 1/π), so the expectation is the
 // reference estimator's for any a and g.
+template <bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+  const pass_hdr H = *P.hdr;
   __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
   __shared__ float bias[G_BIAS];
   __shared__ uint32_t offs[NSUB];
@@ -553,7 +645,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
   float4 *const *out = Q.rq[parity ^ 1u];
   uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const bool lights = S.n_lights > 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < total;
@@ -572,7 +663,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
       const v3 wo = -unit(p.d);
       const float pv[3] = {pt.x, pt.y, pt.z};
-      for (int k = 0; k < 3; ++k) in9[k] = (pv[k] - P.bounds[k]) / fmaxf(P.bounds[3 + k] - P.bounds[k], 1e-20f);
+      const float *bounds = P.scenes[P.jobs[MULTI ? P.pass[p.ent].job : 0u].scene].bounds;
+      for (int k = 0; k < 3; ++k) in9[k] = (pv[k] - bounds[k]) / fmaxf(bounds[3 + k] - bounds[k], 1e-20f);
       in9[3] = n.x, in9[4] = n.y, in9[5] = n.z;
       in9[6] = wo.x, in9[7] = wo.y, in9[8] = wo.z;
       if (P.rec_stride && (p.item % P.rec_stride) == 0 && p.ndiff < REC_VERTS) {
@@ -623,9 +715,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       // a non-finite network output falls back to the reference sampler at this vertex
       const float alpha = finite ? alpha_net : 0.0f;
       const tpt_material M = S.materials[h.mat];
-      const uint32_t px = p.item % P.n_pix, smp = P.s0 + p.item / P.n_pix;
+      const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
+      const scene_view Sj = view_of<MULTI>(S, P, c.job);
+      const bool lights = Sj.n_lights > 0;
       stream R;
-      R.init(px, smp, p.depth + 1, P.seed);
+      R.init(c.px, c.smp, p.depth + 1, P.jobs[c.job].seed);
       const v3 atten = tex_value(S, M.tex, u, v, pt);
       const onb uvw(n);
       v3 nd;
@@ -633,14 +727,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         const float u0 = R.next(), u1 = R.next(), u2 = R.next();
         nd = mix_sample(mix, u0, u1, u2);
       } else if (lights && R.next() < 0.5f) {
-        nd = light_random(S, pt, R);
+        nd = light_random(Sj, pt, R);
       } else {
         nd = uvw.local(on_hemisphere(R));
       }
       const v3 ud = unit(nd);
       // the mixture density first, so the mixture is dead during light_pdf
       const float g = alpha > 0.0f ? mix_pdf(mix, ud) : 0.0f;
-      const float pl = lights ? light_pdf(S, pt, nd) : 0.0f;
+      const float pl = lights ? light_pdf(Sj, pt, nd) : 0.0f;
       st[ST_LIGHT] += lights ? 1u : 0u;
       float pc = dot(ud, uvw.w);
       pc = pc > 0.0f ? pc : 0.0f;
@@ -671,21 +765,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         finished = true;
       }
     }
-    claim_and_append(P, Q, next_count, out, home, p, cont, finished);
+    claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
   }
   block_stats(Q.stats, st, ST_COUNT);
 }
 
-// sum/sumsq += the pass's samples, in sample order per pixel.
+// Each job's sum/sumsq += its samples of the pass, in sample order per pixel.
+template <bool MULTI>
 __global__ void __launch_bounds__(256) k_accumulate(render_params P, const float *__restrict__ res) {
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < P.n_pix; px += gridDim.x * blockDim.x) {
-    float *sum = P.sum + 3ull * px;
+  const pass_hdr H = *P.hdr;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < H.n_acc; t += gridDim.x * blockDim.x) {
+    uint32_t e = 0;
+    if (MULTI) {
+      uint32_t lo = 0, hi = H.n_ent;  // invariant: pass[lo].acc_begin <= t < pass[hi].acc_begin
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.pass[mid].acc_begin <= t) lo = mid; else hi = mid;
+      }
+      e = lo;
+    }
+    const pass_ent E = P.pass[e];
+    const job_dev &J = P.jobs[E.job];
+    const uint32_t px = t - E.acc_begin;
+    float *sum = J.sum + 3ull * px;
     float a0 = sum[0], a1 = sum[1], a2 = sum[2];
     float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
-    float *sq = P.sumsq ? P.sumsq + 3ull * px : nullptr;
+    float *sq = J.sumsq ? J.sumsq + 3ull * px : nullptr;
     if (sq) q0 = sq[0], q1 = sq[1], q2 = sq[2];
-    for (uint32_t s = 0; s < P.pass_spp; ++s) {
-      const float *r = res + 3ull * (static_cast<uint64_t>(s) * P.n_pix + px);
+    for (uint32_t s = 0; s < E.n_layers; ++s) {
+      const float *r = res + 3ull * (E.item_begin + static_cast<uint64_t>(s) * J.n_pix + px);
       const float c0 = r[0], c1 = r[1], c2 = r[2];
       a0 += c0, a1 += c1, a2 += c2;
       q0 += c0 * c0, q1 += c1 * c1, q2 += c2 * c2;

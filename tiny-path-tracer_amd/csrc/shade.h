@@ -137,7 +137,7 @@ __device__ v3 tex_value(const scene_view &S, uint32_t tex, float u, float v, v3 
       continue;
     }
     if (T.kind == TPT_TEX_PERLIN) {
-      const float s = 1.0f + sinf(T.p[0] * p.z + 10.0f * turb(S.perlin, p, T.a));
+      const float s = 1.0f + sinf(T.p[0] * p.z + 10.0f * turb(S.perlin + T.b, p, T.a));  // b: the table of its scene
       const float g = 0.5f * s;
       return v3{g, g, g};
     }

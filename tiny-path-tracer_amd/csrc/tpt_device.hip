@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -84,6 +85,8 @@ bool is_gfx950(int dev) {
 
 }  // namespace
 
+// A scene handle: one or more flattened scenes packed into one pool (indices
+// rebased), plus the render state grown on demand.
 struct tpt_scene {
   int device = 0;
   int n_cu = 256;
@@ -96,35 +99,59 @@ struct tpt_scene {
   uint8_t *texels = nullptr;
   tpt_perlin *perlin = nullptr;
   tpt_light *lights = nullptr;
-  uint32_t n_slots = 0, n_segments = 0, n_lights = 0, light_mode = 0, n_leaves = 0, max_depth = 0;
-  uint32_t n_materials = 0;
-  bool sweep = false;  // wave-uniform traversal (small scenes)
-  float bounds[6] = {0, 0, 0, 1, 1, 1};
+  uint32_t n_slots = 0, n_segments = 0, n_lights = 0;
+  // the pool's scenes
+  std::vector<scene_dev> scn;
+  std::vector<uint8_t> sweep;     // per scene: wave-uniform traversal (small scenes)
+  std::vector<uint32_t> n_slots_of;
+  std::vector<uint32_t> n_leaves_of;
+  scene_dev *d_scenes = nullptr;
   // render state, grown on demand
   void *state = nullptr;
   size_t state_bytes = 0;
   uint32_t capacity = 0;   // path records in flight
   uint64_t res_items = 0;  // items the res buffer holds
-  uint32_t stat_blocks = 0;
+  uint32_t stat_blocks = 0, region_grid = 0, region_guide_grid = 0;
   render_state Q{};
   uint32_t *host_count = nullptr;  // pinned
   std::vector<hipEvent_t> events;  // TPT_RENDER_PROFILE timing
+  // job and pass tables (device) and the pinned staging of the pass table
+  job_dev *d_jobs = nullptr;
+  uint32_t jobs_cap = 0;
+  pass_hdr *d_hdr = nullptr, *h_hdr = nullptr;
+  pass_ent *d_pass = nullptr, *h_pass = nullptr;
+  uint32_t pass_cap = 0;
+  // captured bounce loop (hipGraph), replayed in every pass of a render call
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint8_t> gkey;
 
-  scene_view view() const {
+  // The pooled arrays seen as scene k (its segments and lights).
+  scene_view view(uint32_t k = 0) const {
     scene_view v;
     v.slots = slots;
-    v.segments = segments;
+    v.segments = segments + 2ull * scn[k].seg_begin;
     v.entries = entries;
     v.xforms = xforms;
     v.materials = materials;
     v.textures = textures;
     v.texels = texels;
     v.perlin = perlin;
-    v.lights = lights;
+    v.lights = lights + scn[k].light_begin;
     v.n_slots = n_slots;
+    v.n_segments = scn[k].n_seg;
+    v.n_lights = scn[k].n_lights;
+    v.light_mode = scn[k].light_mode;
+    return v;
+  }
+  // The pooled arrays with every segment and light (multi-job launches).
+  scene_view pool_view() const {
+    scene_view v = view(0);
+    v.segments = segments;
     v.n_segments = n_segments;
+    v.lights = lights;
     v.n_lights = n_lights;
-    v.light_mode = light_mode;
     return v;
   }
 };
@@ -141,36 +168,58 @@ void free_scene(tpt_scene *s) {
   (void)hipFree(s->texels);
   (void)hipFree(s->perlin);
   (void)hipFree(s->lights);
+  (void)hipFree(s->d_scenes);
   (void)hipFree(s->state);
+  (void)hipFree(s->d_jobs);
+  (void)hipFree(s->d_hdr);
+  (void)hipFree(s->d_pass);
+  if (s->h_hdr) (void)hipHostFree(s->h_hdr);
+  if (s->h_pass) (void)hipHostFree(s->h_pass);
   for (auto e : s->events) (void)hipEventDestroy(e);
   if (s->host_count) (void)hipHostFree(s->host_count);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+  if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+  if (s->gstream) (void)hipStreamDestroy(s->gstream);
 }
 
-int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t stat_blocks) {
-  if (s->state && capacity <= s->capacity && res_items <= s->res_items && stat_blocks <= s->stat_blocks)
+// Region size: block b of a grid of G appends to region b % NSUB.  Live
+// records never exceed `capacity`; each block takes grid-stride chunks of 256,
+// so it handles at most t / G + 256 of t entries and the ceil(G / NSUB) blocks
+// of a residue at most t / NSUB + t / G + ceil(G / NSUB) * 256.  Two kernels
+// (grids G1, G2) may append to the same ray queue, their t adding up to at most
+// `capacity`.
+size_t region_size(uint32_t capacity, uint32_t g1, uint32_t g2) {
+  const size_t r1 = (g1 + NSUB - 1) / NSUB, r2 = (g2 + NSUB - 1) / NSUB;
+  return capacity / NSUB + capacity / g1 + capacity / g2 + (r1 + r2) * 256 + 256;
+}
+
+int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t grid, uint32_t guide_grid) {
+  const uint32_t stat_blocks = std::max(grid, guide_grid);
+  if (s->state && capacity <= s->capacity && res_items <= s->res_items && stat_blocks <= s->stat_blocks &&
+      region_size(capacity, grid, guide_grid) <= s->Q.region)
     return TPT_OK;
   capacity = std::max(capacity, s->capacity);
   res_items = std::max(res_items, s->res_items);
-  stat_blocks = std::max(stat_blocks, s->stat_blocks);
+  const uint32_t sb = std::max(stat_blocks, s->stat_blocks);
+  const size_t region = std::max<size_t>(region_size(capacity, grid, guide_grid), s->Q.region);
+  if (region * NSUB > 0xFFFFFFFFull) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue capacity too large");
   (void)hipFree(s->state);
   s->state = nullptr;
   s->capacity = 0;
   s->res_items = 0;
   s->stat_blocks = 0;
-  // Region size: block b appends to region b % NSUB.  Live records never
-  // exceed `capacity`; each block takes grid-stride chunks of 256, so one
-  // block of a grid of G handles at most t / G + 256 of t entries and the
-  // ceil(G / NSUB) blocks of a residue at most t / NSUB + t / G +
-  // ceil(G / NSUB) * 256.  Up to two kernels append to the same ray queue,
-  // with stat_blocks / 4 <= G <= stat_blocks, and their t add up to at most
-  // `capacity`.
-  const size_t per_residue = (stat_blocks + NSUB - 1) / NSUB;
-  const size_t region = capacity / NSUB + 2 * (4 * static_cast<size_t>(capacity) / stat_blocks + per_residue * 256 + 256);
+  s->Q.region = 0;
+  if (s->gexec) {  // the captured loop holds the old queue pointers
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+    s->gkey.clear();
+  }
   const size_t n = region * NSUB;
   // 2 ray queues x 3 planes + 3 material queues x 4 planes, 16 B per entry
   const size_t qbytes = n * 16 * (2 * 3 + 3 * 4) + 40 * 256;  // + rounding of every take()
-  const size_t bytes = 256 + COUNTER_WORDS * sizeof(uint32_t) + stat_blocks * ST_COUNT * sizeof(unsigned long long) +
-                       qbytes + res_items * 3 * sizeof(float) + 256;
+  const size_t bytes = 256 + COUNTER_WORDS * sizeof(uint32_t) + sb * ST_COUNT * sizeof(unsigned long long) + qbytes +
+                       res_items * 3 * sizeof(float) + 256;
   HIP_TRY(hipMalloc(&s->state, bytes));
   s->state_bytes = bytes;
   char *w = static_cast<char *>(s->state);
@@ -182,7 +231,7 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   render_state &Q = s->Q;
   Q.region = static_cast<uint32_t>(region);
   Q.counters = reinterpret_cast<uint32_t *>(take(COUNTER_WORDS * sizeof(uint32_t)));
-  Q.stats = reinterpret_cast<unsigned long long *>(take(stat_blocks * ST_COUNT * sizeof(unsigned long long)));
+  Q.stats = reinterpret_cast<unsigned long long *>(take(sb * ST_COUNT * sizeof(unsigned long long)));
   for (int p = 0; p < 2; ++p)
     for (int k = 0; k < 3; ++k) Q.rq[p][k] = reinterpret_cast<float4 *>(take(n * 16));
   for (int c = 0; c < 3; ++c)
@@ -190,7 +239,191 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t s
   Q.res = reinterpret_cast<float *>(take(res_items * 3 * sizeof(float)));
   s->capacity = capacity;
   s->res_items = res_items;
-  s->stat_blocks = stat_blocks;
+  s->stat_blocks = sb;
+  return TPT_OK;
+}
+
+// Every index the kernels follow must stay inside its array.
+int validate_flat(const tpt_flat_scene *f) {
+  if (!f) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: null scene");
+  if (f->n_segments > 0 && (!f->slots || !f->segments))
+    return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: missing slots/segments");
+  if (f->max_depth > 60) return tpt::fail(TPT_ERR_UNSUPPORTED, "tpt_scene_create: bvh deeper than 60 levels");
+  for (uint32_t g = 0; g < f->n_segments; ++g) {
+    if (f->segments[2 * g] > f->segments[2 * g + 1] || f->segments[2 * g + 1] > f->n_slots)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: segment out of range");
+  }
+  for (uint32_t i = 0; i < f->n_slots; ++i) {
+    const tpt_slot &s = f->slots[i];
+    const uint32_t kind = s.b >> 28;
+    if (kind == TPT_SLOT_BOX) {
+      const uint32_t right = s.b & TPT_PAYLOAD_MASK;
+      if (s.a < i + 2 || s.a > f->n_slots || (right != TPT_NO_RIGHT && (right <= i + 1 || right >= s.a)))
+        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad box links");
+    } else if (kind == TPT_SLOT_LIST) {
+      uint32_t cnt;
+      std::memcpy(&cnt, &s.f[1], 4);
+      if (static_cast<uint64_t>(s.a) + cnt > f->n_entries)
+        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: list entries out of range");
+    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT) {
+      if ((s.a & TPT_PAYLOAD_MASK) >= f->n_materials)
+        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: material index out of range");
+    } else {
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: unknown slot kind");
+    }
+  }
+  for (uint32_t e = 0; e < f->n_entries; ++e) {
+    const tpt_entry &en = f->entries[e];
+    if ((en.a & TPT_PAYLOAD_MASK) >= f->n_materials || (en.xform != 0xFFFFFFFFu && en.xform >= f->n_xforms) ||
+        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT))
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad list entry");
+  }
+  for (uint32_t x = 0; x < f->n_xforms; ++x)
+    if (f->xforms[x].nops > TPT_XFORM_MAX_OPS) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad xform");
+  for (uint32_t m = 0; m < f->n_materials; ++m) {
+    const tpt_material &M = f->materials[m];
+    if ((M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT) && M.tex >= f->n_textures)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: texture index out of range");
+    if (M.kind > TPT_MAT_DIFFUSE_LIGHT) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad material");
+  }
+  for (uint32_t t = 0; t < f->n_textures; ++t) {
+    const tpt_texture &T = f->textures[t];
+    if (T.kind == TPT_TEX_CHECKER && (T.a >= f->n_textures || T.b >= f->n_textures))
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: checker child out of range");
+    if (T.kind == TPT_TEX_PERLIN && (!f->perlin || T.b != 0))
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: perlin tables missing");
+    if (T.kind == TPT_TEX_IMAGE &&
+        (T.b == 0 || T.c == 0 || static_cast<uint64_t>(T.a) + 3ull * T.b * T.c > f->n_texels))
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: image texels out of range");
+    if (T.kind == TPT_TEX_PERLIN && f->perlin) {
+      for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 256; ++i)
+          if (f->perlin->perm[k][i] < 0 || f->perlin->perm[k][i] > 255)
+            return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: perlin permutation out of range");
+    }
+  }
+  if (f->n_lights > 0 && f->light_mode > 2) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad light mode");
+  return TPT_OK;
+}
+
+// Index fields that point into another array, rebased by that array's offset
+// in the pool (the material word keeps its axis/flip bits).
+uint32_t rebase_word(uint32_t word, uint32_t base) {
+  return (word & ~TPT_PAYLOAD_MASK) | ((word & TPT_PAYLOAD_MASK) + base);
+}
+
+int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
+  if (!scenes || !out || n == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: no scenes");
+  *out = nullptr;
+  for (uint32_t k = 0; k < n; ++k) {
+    const int rc = validate_flat(&scenes[k]);
+    if (rc != TPT_OK) return rc;
+  }
+  std::vector<tpt_slot> slots;
+  std::vector<uint32_t> segments;
+  std::vector<tpt_entry> entries;
+  std::vector<tpt_xform> xforms;
+  std::vector<tpt_material> materials;
+  std::vector<tpt_texture> textures;
+  std::vector<uint8_t> texels;
+  std::vector<tpt_perlin> perlin;
+  std::vector<tpt_light> lights;
+  std::vector<scene_dev> scn(n);
+  std::vector<uint8_t> sweep(n);
+  std::vector<uint32_t> nslots(n), nleaves(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    const tpt_flat_scene &f = scenes[k];
+    const uint64_t sb = slots.size(), eb = entries.size(), xb = xforms.size(), mb = materials.size(),
+                   tb = textures.size(), bb = texels.size(), pb = perlin.size(), lb = lights.size(),
+                   gb = segments.size() / 2;
+    if (sb + f.n_slots >= TPT_NO_RIGHT || mb + f.n_materials > TPT_PAYLOAD_MASK || bb + f.n_texels > 0xFFFFFFFFull ||
+        eb + f.n_entries > 0xFFFFFFF0ull)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: pool too large for 32-bit indices");
+    for (uint32_t i = 0; i < f.n_slots; ++i) {
+      tpt_slot s = f.slots[i];
+      const uint32_t kind = s.b >> 28;
+      if (kind == TPT_SLOT_BOX) {
+        s.a += static_cast<uint32_t>(sb);
+        const uint32_t right = s.b & TPT_PAYLOAD_MASK;
+        if (right != TPT_NO_RIGHT) s.b = (s.b & ~TPT_PAYLOAD_MASK) | (right + static_cast<uint32_t>(sb));
+      } else if (kind == TPT_SLOT_LIST) {
+        s.a += static_cast<uint32_t>(eb);
+      } else {
+        s.a = rebase_word(s.a, static_cast<uint32_t>(mb));
+      }
+      slots.push_back(s);
+    }
+    for (uint32_t g = 0; g < 2 * f.n_segments; ++g) segments.push_back(f.segments[g] + static_cast<uint32_t>(sb));
+    for (uint32_t e = 0; e < f.n_entries; ++e) {
+      tpt_entry en = f.entries[e];
+      if (en.xform != 0xFFFFFFFFu) en.xform += static_cast<uint32_t>(xb);
+      en.a = rebase_word(en.a, static_cast<uint32_t>(mb));
+      entries.push_back(en);
+    }
+    xforms.insert(xforms.end(), f.xforms, f.xforms + f.n_xforms);
+    for (uint32_t m = 0; m < f.n_materials; ++m) {
+      tpt_material M = f.materials[m];
+      if (M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT) M.tex += static_cast<uint32_t>(tb);
+      materials.push_back(M);
+    }
+    for (uint32_t t = 0; t < f.n_textures; ++t) {
+      tpt_texture T = f.textures[t];
+      if (T.kind == TPT_TEX_CHECKER) T.a += static_cast<uint32_t>(tb), T.b += static_cast<uint32_t>(tb);
+      if (T.kind == TPT_TEX_IMAGE) T.a += static_cast<uint32_t>(bb);
+      if (T.kind == TPT_TEX_PERLIN) T.b = static_cast<uint32_t>(pb);
+      textures.push_back(T);
+    }
+    texels.insert(texels.end(), f.texels, f.texels + f.n_texels);
+    if (f.perlin) perlin.push_back(*f.perlin);
+    lights.insert(lights.end(), f.lights, f.lights + f.n_lights);
+    scene_dev &d = scn[k];
+    std::memset(&d, 0, sizeof(d));
+    d.seg_begin = static_cast<uint32_t>(gb);
+    d.n_seg = f.n_segments;
+    d.light_begin = static_cast<uint32_t>(lb);
+    d.n_lights = f.n_lights;
+    d.light_mode = f.n_lights ? f.light_mode : 0;
+    for (int q = 0; q < 6; ++q) d.bounds[q] = f.bounds[q];
+    // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
+    // close to each lane's own count only when the whole tree is small.
+    sweep[k] = f.n_slots <= 64;
+    nslots[k] = f.n_slots;
+    nleaves[k] = f.n_leaves;
+  }
+  if (tpt_device_count() < 1) return tpt::fail(TPT_ERR_NODEVICE, "tpt_scene_create: no gfx950 device");
+  auto *s = new tpt_scene();
+  int rc = TPT_OK;
+  HIP_TRY(hipGetDevice(&s->device));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) s->n_cu = prop.multiProcessorCount;
+  if ((rc = upload(&s->slots, slots.data(), slots.size())) ||
+      (rc = upload(&s->segments, segments.data(), segments.size())) ||
+      (rc = upload(&s->entries, entries.data(), entries.size())) ||
+      (rc = upload(&s->xforms, xforms.data(), xforms.size())) ||
+      (rc = upload(&s->materials, materials.data(), materials.size())) ||
+      (rc = upload(&s->textures, textures.data(), textures.size())) ||
+      (rc = upload(&s->texels, texels.data(), texels.size())) ||
+      (rc = upload(&s->perlin, perlin.data(), perlin.size())) ||
+      (rc = upload(&s->lights, lights.data(), lights.size())) ||
+      (rc = upload(&s->d_scenes, scn.data(), scn.size()))) {
+    free_scene(s);
+    delete s;
+    return rc;
+  }
+  s->n_slots = static_cast<uint32_t>(slots.size());
+  s->n_segments = static_cast<uint32_t>(segments.size() / 2);
+  s->n_lights = static_cast<uint32_t>(lights.size());
+  s->scn = scn;
+  s->sweep = sweep;
+  s->n_slots_of = nslots;
+  s->n_leaves_of = nleaves;
+  if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
+      hipSuccess) {
+    free_scene(s);
+    delete s;
+    return tpt::fail(TPT_ERR_HIP, "tpt_scene_create: hipHostMalloc failed");
+  }
+  *out = s;
   return TPT_OK;
 }
 
@@ -255,98 +488,11 @@ int tpt_device_count(void) {
 
 int tpt_scene_create(const tpt_flat_scene *f, tpt_scene **out) {
   if (!f || !out) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: null argument");
-  *out = nullptr;
-  if (f->n_segments > 0 && (!f->slots || !f->segments))
-    return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: missing slots/segments");
-  if (f->max_depth > 60) return tpt::fail(TPT_ERR_UNSUPPORTED, "tpt_scene_create: bvh deeper than 60 levels");
-  for (uint32_t g = 0; g < f->n_segments; ++g) {
-    if (f->segments[2 * g] > f->segments[2 * g + 1] || f->segments[2 * g + 1] > f->n_slots)
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: segment out of range");
-  }
-  // Every index the kernels follow must stay inside its array.
-  for (uint32_t i = 0; i < f->n_slots; ++i) {
-    const tpt_slot &s = f->slots[i];
-    const uint32_t kind = s.b >> 28;
-    if (kind == TPT_SLOT_BOX) {
-      const uint32_t right = s.b & TPT_PAYLOAD_MASK;
-      if (s.a < i + 2 || s.a > f->n_slots || (right != TPT_NO_RIGHT && (right <= i + 1 || right >= s.a)))
-        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad box links");
-    } else if (kind == TPT_SLOT_LIST) {
-      uint32_t cnt;
-      std::memcpy(&cnt, &s.f[1], 4);
-      if (static_cast<uint64_t>(s.a) + cnt > f->n_entries)
-        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: list entries out of range");
-    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT) {
-      if ((s.a & TPT_PAYLOAD_MASK) >= f->n_materials)
-        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: material index out of range");
-    } else {
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: unknown slot kind");
-    }
-  }
-  for (uint32_t e = 0; e < f->n_entries; ++e) {
-    const tpt_entry &en = f->entries[e];
-    if ((en.a & TPT_PAYLOAD_MASK) >= f->n_materials || (en.xform != 0xFFFFFFFFu && en.xform >= f->n_xforms) ||
-        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT))
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad list entry");
-  }
-  for (uint32_t x = 0; x < f->n_xforms; ++x)
-    if (f->xforms[x].nops > TPT_XFORM_MAX_OPS) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad xform");
-  for (uint32_t m = 0; m < f->n_materials; ++m) {
-    const tpt_material &M = f->materials[m];
-    if ((M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT) && M.tex >= f->n_textures)
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: texture index out of range");
-    if (M.kind > TPT_MAT_DIFFUSE_LIGHT) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad material");
-  }
-  for (uint32_t t = 0; t < f->n_textures; ++t) {
-    const tpt_texture &T = f->textures[t];
-    if (T.kind == TPT_TEX_CHECKER && (T.a >= f->n_textures || T.b >= f->n_textures))
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: checker child out of range");
-    if (T.kind == TPT_TEX_PERLIN && !f->perlin) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: perlin tables missing");
-    if (T.kind == TPT_TEX_IMAGE &&
-        (T.b == 0 || T.c == 0 || static_cast<uint64_t>(T.a) + 3ull * T.b * T.c > f->n_texels))
-      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: image texels out of range");
-    if (T.kind == TPT_TEX_PERLIN && f->perlin) {
-      for (int k = 0; k < 3; ++k)
-        for (int i = 0; i < 256; ++i)
-          if (f->perlin->perm[k][i] < 0 || f->perlin->perm[k][i] > 255)
-            return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: perlin permutation out of range");
-    }
-  }
-  if (f->n_lights > 0 && f->light_mode > 2) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad light mode");
-  if (tpt_device_count() < 1) return tpt::fail(TPT_ERR_NODEVICE, "tpt_scene_create: no gfx950 device");
-  auto *s = new tpt_scene();
-  int rc = TPT_OK;
-  HIP_TRY(hipGetDevice(&s->device));
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) s->n_cu = prop.multiProcessorCount;
-  if ((rc = upload(&s->slots, f->slots, f->n_slots)) || (rc = upload(&s->segments, f->segments, 2ull * f->n_segments)) ||
-      (rc = upload(&s->entries, f->entries, f->n_entries)) || (rc = upload(&s->xforms, f->xforms, f->n_xforms)) ||
-      (rc = upload(&s->materials, f->materials, f->n_materials)) ||
-      (rc = upload(&s->textures, f->textures, f->n_textures)) || (rc = upload(&s->texels, f->texels, f->n_texels)) ||
-      (rc = upload(&s->perlin, f->perlin, f->perlin ? 1 : 0)) || (rc = upload(&s->lights, f->lights, f->n_lights))) {
-    free_scene(s);
-    delete s;
-    return rc;
-  }
-  s->n_slots = f->n_slots;
-  s->n_segments = f->n_segments;
-  s->n_lights = f->n_lights;
-  s->light_mode = f->light_mode;
-  s->n_leaves = f->n_leaves;
-  s->max_depth = f->max_depth;
-  s->n_materials = f->n_materials;
-  // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
-  // close to each lane's own count only when the whole tree is small.
-  s->sweep = f->n_slots <= 64;
-  for (int k = 0; k < 6; ++k) s->bounds[k] = f->bounds[k];
-  if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
-      hipSuccess) {
-    free_scene(s);
-    delete s;
-    return tpt::fail(TPT_ERR_HIP, "tpt_scene_create: hipHostMalloc failed");
-  }
-  *out = s;
-  return TPT_OK;
+  return create_pool(f, 1, out);
+}
+
+int tpt_scene_create_pool(const tpt_flat_scene *scenes, uint32_t n_scenes, tpt_scene **out) {
+  return create_pool(scenes, n_scenes, out);
 }
 
 int tpt_scene_destroy(tpt_scene *s) {
@@ -363,8 +509,8 @@ int tpt_trace_closest_record(tpt_scene *s, const float *d_rays, uint64_t n, floa
   if (n == 0) return TPT_OK;
   const uint64_t blocks = (n + 255) / 256;
   if (blocks > 0x7FFFFFFFull) return tpt::fail(TPT_ERR_INVALID, "tpt_trace_closest: too many rays");
-  const scene_view S = s->view();
-  if (s->sweep)
+  const scene_view S = s->view(0);
+  if (s->sweep[0])
     hipLaunchKernelGGL(k_trace_closest<true>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, n, t_min, t_max, d_leaf, d_t,
                        d_rec, d_mat);
@@ -385,54 +531,201 @@ int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min
 
 namespace {
 
-int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
-                float *d_sumsq, tpt_render_stats *stats, void *stream_ptr) {
-  if (!s || !camera || !desc || !d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
-  if (desc->width == 0 || desc->height == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: empty image");
-  if (desc->flags & ~TPT_RENDER_PROFILE) return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
-  const bool profile = (desc->flags & TPT_RENDER_PROFILE) != 0;
-  if (!guide && desc->guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
-  const uint64_t n_pix = static_cast<uint64_t>(desc->width) * desc->height;
-  if (n_pix > (1ull << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: more than 2^26 pixels");
-  if (static_cast<uint64_t>(desc->sample_offset) + desc->spp > 0xFFFFFFFFull)
-    return tpt::fail(TPT_ERR_INVALID, "tpt_render: sample range overflows");
-  if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (desc->spp == 0) return TPT_OK;
-  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+constexpr uint64_t kPassItems = 1ull << 26;  // res buffer: 768 MB of per-sample results
+constexpr uint32_t kMaxEnt = 0xFFFFu;        // pass entry index shares a word with the depth
 
-  // Pass size: at most 2^26 items (768 MB of per-sample results).
-  const uint64_t max_items = 1ull << 26;
-  const uint32_t pass_spp = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(desc->spp, max_items / n_pix)));
-  const uint64_t pass_items = n_pix * pass_spp;
-  uint32_t cap = desc->queue_capacity ? desc->queue_capacity : (1u << 21);
-  if (cap > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
-  if (cap > pass_items) cap = static_cast<uint32_t>(pass_items);
-  const uint32_t grid = static_cast<uint32_t>(s->n_cu) * 8;
+struct pass_plan {
+  std::vector<pass_ent> ents;
+  uint32_t n_items = 0, n_acc = 0;
+};
+
+// The jobs' sample layers, job after job, packed into passes of at most
+// kPassItems items and kMaxEnt entries (a job has one entry per pass).
+std::vector<pass_plan> plan_passes(const tpt_job *jobs, uint32_t n_jobs) {
+  std::vector<pass_plan> out;
+  pass_plan cur;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const uint32_t npix = jobs[j].width * jobs[j].height;
+    uint32_t s = jobs[j].sample_offset;
+    const uint32_t s_end = jobs[j].sample_offset + jobs[j].spp;
+    while (s < s_end) {
+      const uint64_t room = (kPassItems - cur.n_items) / npix;
+      if (room == 0 || cur.ents.size() >= kMaxEnt) {
+        out.push_back(cur);
+        cur = pass_plan();
+        continue;
+      }
+      const uint32_t take = static_cast<uint32_t>(std::min<uint64_t>(room, s_end - s));
+      pass_ent e{};
+      e.item_begin = cur.n_items;
+      e.job = j;
+      e.s_begin = s;
+      e.n_layers = take;
+      e.acc_begin = cur.n_acc;
+      cur.ents.push_back(e);
+      cur.n_items += take * npix;
+      cur.n_acc += npix;
+      s += take;
+    }
+  }
+  if (!cur.ents.empty()) out.push_back(cur);
+  return out;
+}
+
+template <class T>
+int grow_device(T **p, uint32_t *cap, uint32_t need) {
+  if (need <= *cap) return TPT_OK;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc(reinterpret_cast<void **>(p), static_cast<size_t>(need) * sizeof(T)));
+  *cap = need;
+  return TPT_OK;
+}
+
+struct bounce_launch {
+  scene_view S;
+  render_params P;
+  render_state Q;
+  uint32_t grid, guide_grid;
+  bool sweep, multi, guided;
+};
+
+void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
+  const dim3 g(B.grid), b(256);
+  if (B.sweep)
+    hipLaunchKernelGGL((k_trace<true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+  else if (B.multi)
+    hipLaunchKernelGGL((k_trace<false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+  else
+    hipLaunchKernelGGL((k_trace<false, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+}
+
+void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
+  const dim3 g(B.grid), b(256);
+  if (B.guided) {
+    if (B.multi) {
+      hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+    } else {
+      hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<false>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+    }
+  } else if (B.multi) {
+    hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
+  } else {
+    hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
+  }
+}
+
+// Renders every job; the render loop described in render.h.
+int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch &L, tpt_guide *guide,
+                tpt_render_stats *stats, void *stream_ptr) {
+  if (!s || (n_jobs && !jobs)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
+  if (L.flags & ~TPT_RENDER_PROFILE) return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
+  if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
+  if (L.queue_capacity > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
+  if (L.blocks_per_cu > 64) return tpt::fail(TPT_ERR_INVALID, "tpt_render: blocks_per_cu above 64");
+  if (L.graph_iters > 256) return tpt::fail(TPT_ERR_INVALID, "tpt_render: graph_iters above 256");
+  uint32_t max_depth = 0;
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const tpt_job &J = jobs[j];
+    if (J.scene >= s->scn.size()) return tpt::fail(TPT_ERR_INVALID, "tpt_render: job scene out of range");
+    if (J.width == 0 || J.height == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: empty image");
+    if (static_cast<uint64_t>(J.width) * J.height > kPassItems)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_render: more than 2^26 pixels");
+    if (!J.d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null d_sum");
+    if (J.max_depth > 0xFFFEu) return tpt::fail(TPT_ERR_INVALID, "tpt_render: max_depth above 65534");
+    if (static_cast<uint64_t>(J.sample_offset) + J.spp > 0xFFFFFFFFull)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_render: sample range overflows");
+    max_depth = std::max(max_depth, J.max_depth);
+    total += static_cast<uint64_t>(J.spp) * J.width * J.height;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (total == 0) return TPT_OK;
+  const bool profile = (L.flags & TPT_RENDER_PROFILE) != 0;
+  const bool multi = n_jobs > 1;
+  const bool use_graph = L.graph_iters > 0 && !profile;
+  hipStream_t cs = static_cast<hipStream_t>(stream_ptr);
+
+  // job table
+  std::vector<job_dev> jd(n_jobs);
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const tpt_job &J = jobs[j];
+    job_dev &d = jd[j];
+    std::memset(&d, 0, sizeof(d));
+    d.cam = J.camera;
+    d.width = J.width;
+    d.height = J.height;
+    d.n_pix = J.width * J.height;
+    d.max_depth = J.max_depth;
+    d.seed = J.seed;
+    d.background = J.background;
+    d.scene = J.scene;
+    d.sum = J.d_sum;
+    d.sumsq = J.d_sumsq;
+  }
+  const std::vector<pass_plan> plan = plan_passes(jobs, n_jobs);
+  uint32_t max_items = 0, max_ent = 0;
+  for (const auto &pp : plan) {
+    max_items = std::max(max_items, pp.n_items);
+    max_ent = std::max<uint32_t>(max_ent, static_cast<uint32_t>(pp.ents.size()));
+  }
+  int rc = grow_device(&s->d_jobs, &s->jobs_cap, n_jobs);
+  if (rc != TPT_OK) return rc;
+  if (max_ent > s->pass_cap) {
+    if (s->h_pass) (void)hipHostFree(s->h_pass);
+    s->h_pass = nullptr;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&s->h_pass), max_ent * sizeof(pass_ent), hipHostMallocDefault));
+    if ((rc = grow_device(&s->d_pass, &s->pass_cap, max_ent)) != TPT_OK) return rc;
+  }
+  if (!s->d_hdr) {
+    HIP_TRY(hipMalloc(&s->d_hdr, sizeof(pass_hdr)));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&s->h_hdr), sizeof(pass_hdr), hipHostMallocDefault));
+  }
+  if (use_graph && !s->gstream) {
+    HIP_TRY(hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming));
+  }
+  hipStream_t ws = use_graph ? s->gstream : cs;
+  if (use_graph) {
+    HIP_TRY(hipEventRecord(s->ev_in, cs));
+    HIP_TRY(hipStreamWaitEvent(ws, s->ev_in, 0));
+  }
+  HIP_TRY(hipMemcpyAsync(s->d_jobs, jd.data(), n_jobs * sizeof(job_dev), hipMemcpyHostToDevice, ws));
+
+  const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 8;
+  const uint32_t grid = static_cast<uint32_t>(s->n_cu) * bpc;
   // k_guide_shade holds 51 KB of LDS (weights + output table): 3 blocks per CU
   const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * 3;
-  int rc = ensure_state(s, cap, pass_items, grid);
-  if (rc != TPT_OK) return rc;
+  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 21);
+  cap = std::min(cap, max_items);
+  if ((rc = ensure_state(s, cap, max_items, grid, guide_grid)) != TPT_OK) return rc;
 
-  render_params P{};
-  P.cam = *camera;
-  P.width = desc->width;
-  P.height = desc->height;
-  P.n_pix = static_cast<uint32_t>(n_pix);
-  P.max_depth = desc->max_depth;
-  P.seed = desc->seed;
-  P.background = desc->background;
-  P.sum = d_sum;
-  P.sumsq = d_sumsq;
-  const scene_view S = s->view();
-  const render_state &Q = s->Q;
-  P.guide_verts = guide ? (desc->guide_vertices ? desc->guide_vertices : 2u) : ~0u;
+  bounce_launch B;
+  B.S = multi ? s->pool_view() : s->view(jobs[0].scene);
+  B.Q = s->Q;
+  B.grid = grid;
+  B.guide_grid = guide_grid;
+  B.sweep = !multi && s->sweep[jobs[0].scene];
+  B.multi = multi;
+  B.guided = guide != nullptr;
+  render_params &P = B.P;
+  P = render_params{};
+  P.jobs = s->d_jobs;
+  P.scenes = s->d_scenes;
+  P.hdr = s->d_hdr;
+  P.pass = s->d_pass;
+  P.guide_verts = ~0u;
   if (guide) {
     P.g_tiles = guide->xtiles;
     P.g_params = guide->params;
-    for (int k = 0; k < 6; ++k) P.bounds[k] = s->bounds[k];
-    P.rec_stride = desc->guiding;
+    P.guide_verts = L.guide_vertices ? L.guide_vertices : 2u;
+    P.rec_stride = L.guiding;
     if (P.rec_stride) {
-      const uint64_t need = pass_items / P.rec_stride + 1;
+      const uint64_t need = max_items / P.rec_stride + 1;
       if (need > guide->rec_cap) {
         (void)hipFree(guide->rec_side);
         (void)hipFree(guide->rec_rows);
@@ -444,16 +737,44 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
       }
       P.rec_side = guide->rec_side;
       P.rec_rows = guide->rec_rows;
-      guide->n_rows = (std::min<uint64_t>(pass_items, static_cast<uint64_t>(n_pix) * desc->spp) + P.rec_stride - 1) /
-                      P.rec_stride * REC_VERTS;
+      guide->n_rows = (static_cast<uint64_t>(max_items) + P.rec_stride - 1) / P.rec_stride * REC_VERTS;
+    }
+  }
+
+  // The captured bounce loop: graph_iters iterations (even, so a replay
+  // starts and ends at parity 0), kept while nothing it captured changes.
+  const uint32_t K = use_graph ? (L.graph_iters + 1) / 2 * 2 : 0;
+  if (use_graph) {
+    std::vector<uint8_t> key(sizeof(bounce_launch) + sizeof(K));
+    std::memcpy(key.data(), &B, sizeof(bounce_launch));
+    std::memcpy(key.data() + sizeof(bounce_launch), &K, sizeof(K));
+    if (!s->gexec || key != s->gkey) {
+      if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+      s->gexec = nullptr;
+      s->gkey.clear();
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+      for (uint32_t it = 0; it < K; ++it) {
+        launch_trace(B, ws, it & 1u);
+        launch_shade(B, ws, it & 1u);
+      }
+      HIP_TRY(hipStreamEndCapture(ws, &graph));
+      const hipError_t ie = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        s->gexec = nullptr;
+        return tpt::fail(TPT_ERR_HIP, std::string("tpt_render: hipGraphInstantiate: ") + hipGetErrorString(ie));
+      }
+      s->gkey = key;
     }
   }
 
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(hipEventRecord(e0, stream));
-  hipLaunchKernelGGL(k_zero_stats, dim3(64), dim3(256), 0, stream, Q.stats, grid * ST_COUNT);
+  HIP_TRY(hipEventRecord(e0, ws));
+  const uint32_t n_stat = std::max(grid, guide_grid);  // rows written by block_stats
+  hipLaunchKernelGGL(k_zero_stats, dim3(64), dim3(256), 0, ws, s->Q.stats, n_stat * ST_COUNT);
   const int poll = 16;
   if (profile && s->events.size() < static_cast<size_t>(2 * poll + 1)) {
     for (auto e : s->events) (void)hipEventDestroy(e);
@@ -470,62 +791,88 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
     }
     n_launch += iters;
   };
+  // live records of the next input queue (parity p)
+  auto live_count = [&](uint32_t p, uint64_t *live) -> int {
+    HIP_TRY(hipMemcpyAsync(s->host_count, s->Q.counters + rc_at(p, 0), NSUB * CSTRIDE * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ws));
+    HIP_TRY(hipStreamSynchronize(ws));
+    *live = 0;
+    for (uint32_t k = 0; k < NSUB; ++k) *live += s->host_count[k * CSTRIDE];
+    return TPT_OK;
+  };
   bool drained = true;
-  for (uint32_t done = 0; done < desc->spp && drained; done += P.pass_spp) {
-    P.s0 = desc->sample_offset + done;
-    P.pass_spp = std::min(pass_spp, desc->spp - done);
-    P.n_items = static_cast<uint32_t>(n_pix * P.pass_spp);
-    P.n_first = std::min(cap, P.n_items);
-    hipLaunchKernelGGL(k_init, dim3(std::min<uint32_t>((P.n_first + 255) / 256 + 1, grid)), dim3(256), 0, stream, P,
-                       Q);
+  for (const pass_plan &pp : plan) {
+    if (!drained) break;
+    // The staging copies of the previous pass completed at its last poll.
+    s->h_hdr->n_items = pp.n_items;
+    s->h_hdr->n_first = std::min(cap, pp.n_items);
+    s->h_hdr->n_ent = static_cast<uint32_t>(pp.ents.size());
+    s->h_hdr->n_acc = pp.n_acc;
+    std::memcpy(s->h_pass, pp.ents.data(), pp.ents.size() * sizeof(pass_ent));
+    HIP_TRY(hipMemcpyAsync(s->d_hdr, s->h_hdr, sizeof(pass_hdr), hipMemcpyHostToDevice, ws));
+    HIP_TRY(hipMemcpyAsync(s->d_pass, s->h_pass, pp.ents.size() * sizeof(pass_ent), hipMemcpyHostToDevice, ws));
+    const uint32_t n_first = s->h_hdr->n_first;
+    const dim3 ig(std::min<uint32_t>((n_first + 255) / 256 + 1, grid));
+    if (multi)
+      hipLaunchKernelGGL(k_init<true>, ig, dim3(256), 0, ws, P, s->Q);
+    else
+      hipLaunchKernelGGL(k_init<false>, ig, dim3(256), 0, ws, P, s->Q);
     // A path has at most max_depth + 1 segments and each bounce retires at
     // least one path per live record, so the loop is bounded.
-    const uint64_t max_iter = (static_cast<uint64_t>(P.n_items) / cap + 2) * (desc->max_depth + 2ull) + 64;
+    const uint64_t max_iter = (static_cast<uint64_t>(pp.n_items) / cap + 2) * (max_depth + 2ull) + 64;
     drained = false;
-    int batch = 0;
-    for (uint64_t it = 0; it < max_iter; ++it) {
-      const uint32_t parity = static_cast<uint32_t>(it & 1u);
-      if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch], stream));
-      if (s->sweep)
-        hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
-      else
-        hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(256), 0, stream, S, S.slots, S.entries, P, Q, parity);
-      if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 1], stream));
-      if (guide) {
-        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
-        hipLaunchKernelGGL(k_guide_shade, dim3(guide_grid), dim3(256), 0, stream, S, P, Q, parity);
-      } else {
-        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, stream, S, P, Q, parity);
-      }
-      if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 2], stream));
-      ++batch;
-      if ((it % poll) == poll - 1) {
-        HIP_TRY(hipMemcpyAsync(s->host_count, Q.counters + rc_at(parity ^ 1u, 0), NSUB * CSTRIDE * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        if (profile) harvest(batch);
-        batch = 0;
+    if (use_graph) {
+      for (uint64_t it = 0; it < max_iter; it += K) {
+        HIP_TRY(hipGraphLaunch(s->gexec, ws));
         uint64_t live = 0;
-        for (uint32_t k = 0; k < NSUB; ++k) live += s->host_count[k * CSTRIDE];
+        if ((rc = live_count(0, &live)) != TPT_OK) return rc;
         if (live == 0) {
           drained = true;
           break;
         }
       }
-    }
-    if (profile && batch > 0) {
-      HIP_TRY(hipStreamSynchronize(stream));
-      harvest(batch);
+    } else {
+      int batch = 0;
+      for (uint64_t it = 0; it < max_iter; ++it) {
+        const uint32_t parity = static_cast<uint32_t>(it & 1u);
+        if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch], ws));
+        launch_trace(B, ws, parity);
+        if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 1], ws));
+        launch_shade(B, ws, parity);
+        if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 2], ws));
+        ++batch;
+        if ((it % poll) == poll - 1) {
+          uint64_t live = 0;
+          if ((rc = live_count(parity ^ 1u, &live)) != TPT_OK) return rc;
+          if (profile) harvest(batch);
+          batch = 0;
+          if (live == 0) {
+            drained = true;
+            break;
+          }
+        }
+      }
+      if (profile && batch > 0) {
+        HIP_TRY(hipStreamSynchronize(ws));
+        harvest(batch);
+      }
     }
     if (!drained) break;
-    hipLaunchKernelGGL(k_accumulate, dim3(std::min<uint32_t>((P.n_pix + 255) / 256, grid)), dim3(256), 0, stream, P,
-                       static_cast<const float *>(Q.res));
+    const dim3 ag(std::min<uint32_t>((pp.n_acc + 255) / 256, grid));
+    if (multi)
+      hipLaunchKernelGGL(k_accumulate<true>, ag, dim3(256), 0, ws, P, static_cast<const float *>(s->Q.res));
+    else
+      hipLaunchKernelGGL(k_accumulate<false>, ag, dim3(256), 0, ws, P, static_cast<const float *>(s->Q.res));
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(e1, stream));
-  std::vector<unsigned long long> st(static_cast<size_t>(grid) * ST_COUNT);
-  HIP_TRY(hipMemcpyAsync(st.data(), Q.stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
+  HIP_TRY(hipEventRecord(e1, ws));
+  std::vector<unsigned long long> st(static_cast<size_t>(n_stat) * ST_COUNT);
+  HIP_TRY(hipMemcpyAsync(st.data(), s->Q.stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, ws));
+  HIP_TRY(hipStreamSynchronize(ws));
+  if (use_graph) {
+    HIP_TRY(hipEventRecord(s->ev_out, ws));
+    HIP_TRY(hipStreamWaitEvent(cs, s->ev_out, 0));
+  }
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
@@ -533,7 +880,7 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
   if (!drained) return tpt::fail(TPT_ERR_HIP, "tpt_render: bounce loop did not drain");
   if (stats) {
     unsigned long long tot[ST_COUNT] = {0};
-    for (uint32_t b = 0; b < grid; ++b)
+    for (uint32_t b = 0; b < n_stat; ++b)
       for (int k = 0; k < ST_COUNT; ++k) tot[k] += st[static_cast<size_t>(b) * ST_COUNT + k];
     stats->paths = tot[ST_PATHS];
     stats->segments = tot[ST_SEGMENTS];
@@ -549,20 +896,112 @@ int render_impl(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const 
   return TPT_OK;
 }
 
+int render_single(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+                  float *d_sumsq, tpt_render_stats *stats, void *stream) {
+  if (!s || !camera || !desc || !d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
+  tpt_job J;
+  std::memset(&J, 0, sizeof(J));
+  J.scene = 0;
+  J.width = desc->width;
+  J.height = desc->height;
+  J.spp = desc->spp;
+  J.max_depth = desc->max_depth;
+  J.sample_offset = desc->sample_offset;
+  J.seed = desc->seed;
+  J.background = desc->background;
+  J.camera = *camera;
+  J.d_sum = d_sum;
+  J.d_sumsq = d_sumsq;
+  tpt_launch L;
+  std::memset(&L, 0, sizeof(L));
+  L.queue_capacity = desc->queue_capacity;
+  L.flags = desc->flags;
+  L.guiding = desc->guiding;
+  L.guide_vertices = desc->guide_vertices;
+  return render_core(s, &J, 1, L, guide, stats, stream);
+}
+
 }  // namespace
 
 extern "C" {
 
 int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum, float *d_sumsq,
                tpt_render_stats *stats, void *stream) {
-  return render_impl(s, nullptr, camera, desc, d_sum, d_sumsq, stats, stream);
+  return render_single(s, nullptr, camera, desc, d_sum, d_sumsq, stats, stream);
 }
 
 int tpt_render_guided(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, const tpt_render_desc *desc,
                       float *d_sum, float *d_sumsq, tpt_render_stats *stats, void *stream) {
   if (!guide) return tpt::fail(TPT_ERR_INVALID, "tpt_render_guided: null guide");
-  return render_impl(s, guide, camera, desc, d_sum, d_sumsq, stats, stream);
+  return render_single(s, guide, camera, desc, d_sum, d_sumsq, stats, stream);
 }
+
+int tpt_render_jobs(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *launch, tpt_guide *guide,
+                    tpt_render_stats *stats, void *stream) {
+  tpt_launch L;
+  std::memset(&L, 0, sizeof(L));
+  if (launch) L = *launch;
+  return render_core(s, jobs, n_jobs, L, guide, stats, stream);
+}
+
+int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *guide, uint32_t calib_spp,
+                 tpt_launch *best, float *best_ms) {
+  if (!s || !jobs || n_jobs == 0 || !best) return tpt::fail(TPT_ERR_INVALID, "tpt_autotune: bad argument");
+  const uint32_t cs_ = calib_spp ? calib_spp : 2;
+  // calibration jobs: at most cs_ samples each, into scratch sums
+  std::vector<tpt_job> cj(jobs, jobs + n_jobs);
+  std::vector<float *> scratch(n_jobs, nullptr);
+  auto release = [&]() {
+    for (float *p : scratch) (void)hipFree(p);
+  };
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    if (hipMalloc(&scratch[j], 3ull * jobs[j].width * jobs[j].height * sizeof(float)) != hipSuccess) {
+      release();
+      return tpt::fail(TPT_ERR_NOMEM, "tpt_autotune: scratch allocation failed");
+    }
+    cj[j].spp = std::min(jobs[j].spp, cs_);
+    cj[j].d_sum = scratch[j];
+    cj[j].d_sumsq = nullptr;
+  }
+  const uint32_t caps[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+  const uint32_t bpcs[] = {4, 8, 16};
+  const uint32_t graphs[] = {0, 8};
+  tpt_launch base = *best;
+  base.flags = 0;
+  float best_t = 3.0e38f;
+  tpt_launch pick = base;
+  for (uint32_t g : graphs)
+    for (uint32_t c : caps)
+      for (uint32_t b : bpcs) {
+        tpt_launch L = base;
+        L.queue_capacity = c;
+        L.blocks_per_cu = b;
+        L.graph_iters = g;
+        L.guiding = 0;  // calibration renders do not record training rows
+        float t = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {  // the first run warms up (state growth, graph capture)
+          const auto t0 = std::chrono::steady_clock::now();
+          const int rc = render_core(s, cj.data(), n_jobs, L, guide, nullptr, nullptr);
+          if (rc != TPT_OK) {
+            release();
+            return rc;
+          }
+          t = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (t < best_t) {
+          best_t = t;
+          pick = L;
+        }
+      }
+  release();
+  pick.flags = best->flags;
+  pick.guiding = best->guiding;
+  pick.guide_vertices = best->guide_vertices;
+  *best = pick;
+  if (best_ms) *best_ms = best_t;
+  return TPT_OK;
+}
+
 
 int tpt_guide_create(uint64_t seed, tpt_guide **out) {
   if (!out) return tpt::fail(TPT_ERR_INVALID, "tpt_guide_create: null argument");
@@ -707,3 +1146,4 @@ int tpt_guide_batch(tpt_guide *g, const float **d_rows, uint64_t *n_rows) {
 }
 
 }  // extern "C"
+
