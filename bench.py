@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--scene", default="cornell555")
     p.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     p.add_argument("--ref-spp", type=int, default=65536, help="spp of the converged plain render RMSE is taken against")
+    p.add_argument("--workload", default="cornell", choices=["cornell", "batch64"],
+                   help="cornell: the headline frame; batch64: 64 uneven random-sphere renders in one batch")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run one 256 MiB device copy first (the known-size kernel for HBM counter scaling)")
     return p.parse_args()
@@ -144,6 +146,114 @@ def cpu_baseline(scene, w, h, spp, depth):
                       f"(one std::async task per image row)"}
 
 
+BATCH_SEED = 2026
+
+
+def batch64_specs(n=64, seed=BATCH_SEED):
+    """The uneven batch: sphere counts log-uniform in [50, 20000], resolution
+    uniform in {128, 256, 512, 1024}^2, spp uniform in {4, 16, 64, 256}, depth 50."""
+    rng = np.random.default_rng(seed)
+    counts = np.round(np.exp(rng.uniform(np.log(50), np.log(20000), n))).astype(int)
+    res = rng.choice([128, 256, 512, 1024], n)
+    spp = rng.choice([4, 16, 64, 256], n)
+    return [dict(spheres=int(c), res=int(r), spp=int(s)) for c, r, s in zip(counts, res, spp)]
+
+
+def batch64(tpt, torch, args):
+    """One step = the whole batch rendered by tpt_render_jobs with the
+    autotuned launch; beside it, the same renders one tpt_render call each, and
+    the batch with the guide on (one network for every scene, trained online
+    on the batch's own rows first)."""
+    specs = batch64_specs()
+    t0 = time.perf_counter()
+    hosts = [tpt.HostScene(f"random_spheres_n{sp['spheres']}", seed=k + 1) for k, sp in enumerate(specs)]
+    pool = tpt.Scene.pool(hosts)
+    t_build = time.perf_counter() - t0
+
+    def make_jobs(seed):
+        jobs, bufs = [], []
+        for k, sp in enumerate(specs):
+            j, s_, q_ = pool.job(scene=k, width=sp["res"], height=sp["res"], spp=sp["spp"], seed=seed + k,
+                                 max_depth=50, sumsq_buf=False)
+            jobs.append(j)
+            bufs.append(s_)
+        return jobs, bufs
+
+    jobs, bufs = make_jobs(1)
+    t0 = time.perf_counter()
+    launch, calib_ms = pool.autotune(jobs, calib_spp=1)
+    t_tune = time.perf_counter() - t0
+    samples = sum(sp["res"] ** 2 * sp["spp"] for sp in specs)
+    for _ in range(args.warmup):
+        pool.render_jobs(jobs, launch=launch)
+    torch.cuda.synchronize()
+    segs = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        jobs, bufs = make_jobs(100 + 64 * k)
+        st = pool.render_jobs(jobs, launch=launch)
+        segs += st["segments"]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the same renders one call each (one job per tpt_render_jobs call, the
+    # tpt_render path; default launch settings)
+    jobs, bufs = make_jobs(3000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in jobs:
+        pool.render_jobs([j])
+    torch.cuda.synchronize()
+    dt_single = time.perf_counter() - t0
+    # guided batch: short online training on the batch (1-spp passes recording
+    # every 16th path), then the timed batch with the trained guide
+    g = tpt.Guide(seed=1)
+    gl = tpt.Launch()
+    gl.queue_capacity, gl.blocks_per_cu, gl.graph_iters = launch.queue_capacity, launch.blocks_per_cu, launch.graph_iters
+    tl = tpt.Launch()
+    tl.queue_capacity, tl.blocks_per_cu, tl.graph_iters, tl.guiding = gl.queue_capacity, gl.blocks_per_cu, 0, 16
+    t0 = time.perf_counter()
+    for r in range(4):
+        tj, _ = make_jobs(5000 + 64 * r)
+        for j in tj:
+            j.spp = 1
+        pool.render_jobs(tj, launch=tl, guide=g)
+        g.train(None, n_batch=1 << 18, iters=32, lr=5e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    jobs, bufs = make_jobs(7000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pool.render_jobs(jobs, launch=gl, guide=g)
+    torch.cuda.synchronize()
+    dt_guided = time.perf_counter() - t0
+    value = samples / dt / 1e6
+    return {
+        "metric": "Msamples/s (batched renders: 64 random-sphere scenes, uneven resolutions and spp, one batch)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic (64 procedural random-sphere scenes, seed {BATCH_SEED})",
+        "config": {"workload": "batch64: 64 random_spheres_n<N> scenes, N log-uniform in [50, 20000], "
+                               "res uniform in {128,256,512,1024}^2, spp uniform in {4,16,64,256}, depth 50",
+                   "samples": samples, "scenes": [sp["spheres"] for sp in specs]},
+        "detail": {"Mrays_per_s": round(segs / args.steps / dt / 1e6, 2),
+                   "autotuned_launch": launch.as_dict(), "autotune_s": round(t_tune, 2),
+                   "calibration_ms": round(calib_ms, 2), "scene_build_s": round(t_build, 2),
+                   "one_call_per_render_ms": round(dt_single * 1e3, 2),
+                   "batch_speedup_vs_one_call_per_render": round(dt_single / dt, 3),
+                   "guided_batch_ms": round(dt_guided * 1e3, 2),
+                   "guided_Msamples_per_s": round(samples / dt_guided / 1e6, 3),
+                   "guide_training_s": round(t_train, 3)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -153,6 +263,9 @@ def main():
     import torch
 
     tpt = importlib.import_module("tiny-path-tracer_amd")
+    if args.workload == "batch64":
+        print(json.dumps(batch64(tpt, torch, args)), flush=True)
+        return
     if args.copy_calibration:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
