@@ -209,6 +209,8 @@ typedef struct tpt_scene_info {
   uint32_t width, height, spp, max_depth; /* the scene's default render settings */
   float background;                       /* sky scale used when the scene has no light */
   uint32_t n_leaves;
+  float lookfrom[3], lookat[3];           /* its camera: vup (0,1,0), vfov degrees, aperture, focus */
+  float vfov, aperture, focus;
 } tpt_scene_info;
 
 const char *tpt_scene_names(void);
@@ -217,6 +219,13 @@ int tpt_host_scene_flat(const tpt_host_scene *hs, tpt_flat_scene *flat);
 int tpt_host_scene_camera(const tpt_host_scene *hs, tpt_camera *camera);
 int tpt_host_scene_info(const tpt_host_scene *hs, tpt_scene_info *info);
 int tpt_host_scene_destroy(tpt_host_scene *hs);
+
+/* The fields of camera_with_blur(lookfrom, lookat, vup, vfov, aspect,
+ * aperture, focus_dist, t0, t1) as its constructor computes them
+ * (src/camera.cc:2-21): for a driver that sets fov, aspect, aperture and the
+ * shutter from its configuration (main.cpp:87-91). */
+int tpt_camera_make(const float lookfrom[3], const float lookat[3], const float vup[3], float vfov, float aspect,
+                    float aperture, float focus_dist, float t0, float t1, tpt_camera *out);
 
 /* ------------------------------------------------------------- lifecycle --- */
 typedef struct tpt_scene tpt_scene;

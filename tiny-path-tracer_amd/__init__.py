@@ -68,7 +68,8 @@ class Camera(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
-                ("background", C.c_float), ("n_leaves", C.c_uint32)]
+                ("background", C.c_float), ("n_leaves", C.c_uint32), ("lookfrom", C.c_float * 3),
+                ("lookat", C.c_float * 3), ("vfov", C.c_float), ("aperture", C.c_float), ("focus", C.c_float)]
 
 
 class RenderDesc(C.Structure):
@@ -126,6 +127,7 @@ def lib():
     L.tpt_host_scene_camera.argtypes = [vp, C.POINTER(Camera)]
     L.tpt_host_scene_info.argtypes = [vp, C.POINTER(SceneInfo)]
     L.tpt_host_scene_destroy.argtypes = [vp]
+    L.tpt_camera_make.argtypes = [vp, vp, vp, f32, f32, f32, f32, f32, f32, C.POINTER(Camera)]
     L.tpt_scene_create.argtypes = [C.POINTER(FlatScene), C.POINTER(vp)]
     L.tpt_scene_destroy.argtypes = [vp]
     L.tpt_trace_closest.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp]
@@ -459,6 +461,16 @@ def display_rmse(sum_buf, spp, reference_display):
     torch = _torch()
     d = torch.clamp(torch.sqrt(sum_buf / spp), 0.0, 1.0)
     return torch.sqrt(torch.mean((d - reference_display) ** 2)).item()
+
+
+def camera_make(lookfrom, lookat, vup=(0.0, 1.0, 0.0), vfov=90.0, aspect=1.0, aperture=0.0, focus=1.0, t0=0.0,
+                t1=0.0):
+    """tpt_camera_make: the fields of camera_with_blur(...) (src/camera.cc:2-21)."""
+    f3 = C.c_float * 3
+    cam = Camera()
+    _check(lib().tpt_camera_make(f3(*lookfrom), f3(*lookat), f3(*vup), vfov, aspect, aperture, focus, t0, t1,
+                                 C.byref(cam)), "tpt_camera_make")
+    return cam
 
 
 def tonemap_ppm(mean_rgb):

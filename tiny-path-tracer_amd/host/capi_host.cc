@@ -70,6 +70,20 @@ int tpt_host_scene_info(const tpt_host_scene *hs, tpt_scene_info *info) {
   info->max_depth = hs->spec.max_depth;
   info->background = hs->spec.background;
   info->n_leaves = hs->flat.n_leaves;
+  for (int k = 0; k < 3; ++k) info->lookfrom[k] = hs->spec.lookfrom[k], info->lookat[k] = hs->spec.lookat[k];
+  info->vfov = hs->spec.vfov;
+  info->aperture = hs->spec.aperture;
+  info->focus = hs->spec.focus;
+  return TPT_OK;
+}
+
+int tpt_camera_make(const float lookfrom[3], const float lookat[3], const float vup[3], float vfov, float aspect,
+                    float aperture, float focus_dist, float t0, float t1, tpt_camera *out) {
+  if (!lookfrom || !lookat || !vup || !out) return tpt::fail(TPT_ERR_INVALID, "tpt_camera_make: null argument");
+  if (!(aspect > 0.0f)) return tpt::fail(TPT_ERR_INVALID, "tpt_camera_make: aspect must be positive");
+  const camera_with_blur cam(vec3(lookfrom[0], lookfrom[1], lookfrom[2]), vec3(lookat[0], lookat[1], lookat[2]),
+                             vec3(vup[0], vup[1], vup[2]), vfov, aspect, aperture, focus_dist, t0, t1);
+  *out = tpt::camera_fields(cam);
   return TPT_OK;
 }
 

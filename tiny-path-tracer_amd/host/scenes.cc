@@ -45,6 +45,10 @@ namespace {
 
 void make_camera(scene_spec *s, vec3 from, vec3 at, float vfov, float aperture, float focus) {
   const float aspect = static_cast<float>(s->width) / static_cast<float>(s->height);
+  for (int k = 0; k < 3; ++k) s->lookfrom[k] = from[k], s->lookat[k] = at[k];
+  s->vfov = vfov;
+  s->aperture = aperture;
+  s->focus = focus;
   s->cam.reset(new camera_with_blur(from, at, vec3(0, 1, 0), vfov, aspect, aperture, focus, 0.0f, 0.0f));
 }
 

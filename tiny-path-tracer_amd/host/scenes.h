@@ -21,6 +21,8 @@ struct scene_spec {
   std::unique_ptr<camera_with_blur> cam;
   uint32_t width = 0, height = 0, spp = 0, max_depth = 50;
   float background = 0.0f;
+  // the camera's construction parameters (a driver rebuilds it with its own fov / aspect)
+  float lookfrom[3] = {0, 0, 0}, lookat[3] = {0, 0, -1}, vfov = 90.0f, aperture = 0.0f, focus = 1.0f;
   std::vector<unsigned char> image;  // texels owned for image_texture
 };
 
