@@ -44,8 +44,9 @@ def parse():
     p.add_argument("--scene", default="cornell555")
     p.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     p.add_argument("--ref-spp", type=int, default=65536, help="spp of the converged plain render RMSE is taken against")
-    p.add_argument("--workload", default="cornell", choices=["cornell", "batch64"],
-                   help="cornell: the headline frame; batch64: 64 uneven random-sphere renders in one batch")
+    p.add_argument("--workload", default="cornell", choices=["cornell", "batch64", "guide_train"],
+                   help="cornell: the headline frame; batch64: 64 uneven random-sphere renders in one batch; "
+                        "guide_train: Adam steps of the guiding network on 2^18 rows")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run one 256 MiB device copy first (the known-size kernel for HBM counter scaling)")
     return p.parse_args()
@@ -254,6 +255,58 @@ def batch64(tpt, torch, args):
     }
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+TRAIN_FLOP_PER_ROW = 80640  # forward 26,880 + backward data and weights (2x)
+
+
+def guide_train(tpt, torch, args):
+    """One step = one Adam step of the guiding network on 2^18 training rows
+    (fused forward/backward, reduction, Adam, tile packing), on synthetic
+    rows of the recorded layout, cycling through 2^20 of them."""
+    rng = np.random.default_rng(7)
+    n = 1 << 20
+    rows = np.zeros((n, 16), np.float32)
+    rows[:, 0:3] = rng.uniform(0, 1, (n, 3))
+    for c in (3, 6, 9):
+        v = rng.normal(size=(n, 3))
+        rows[:, c:c + 3] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    rows[:, 12] = rng.exponential(1.0, n)
+    rows[:, 13] = rng.uniform(0.05, 1.0, n)
+    rt = torch.from_numpy(rows).cuda()
+    g = tpt.Guide(seed=1)
+    batch = 1 << 18
+    steps = max(args.steps, 1) * 100
+    g.train(rt, n_batch=batch, iters=max(args.warmup, 1) * 10, lr=1e-3)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    loss = g.train(rt, n_batch=batch, iters=steps, lr=1e-3)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    tflops = batch * TRAIN_FLOP_PER_ROW / (ms / 1e3) / 1e12
+    return {
+        "metric": "training rows/s of the guiding network (Adam steps of 2^18 rows)",
+        "value": round(batch / (ms / 1e3) / 1e6, 3),
+        "unit": "Mrows/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(args.warmup, 1) * 10,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic training rows (uniform positions, random unit vectors, exponential contributions)",
+        "config": {"workload": "guide_train: 42->3x64->40 vMF-mixture network, batch 2^18 rows, 2^20 rows cycled",
+                   "batch": batch},
+        "roofline": {"bound": "mfma", "kernel": "k_guide_train + reduce + Adam + pack", "achieved": round(tflops, 2),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5),
+                     "traffic": None, "flop_per_row": TRAIN_FLOP_PER_ROW},
+        "detail": {"last_loss": round(loss, 5)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -265,6 +318,9 @@ def main():
     tpt = importlib.import_module("tiny-path-tracer_amd")
     if args.workload == "batch64":
         print(json.dumps(batch64(tpt, torch, args)), flush=True)
+        return
+    if args.workload == "guide_train":
+        print(json.dumps(guide_train(tpt, torch, args)), flush=True)
         return
     if args.copy_calibration:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
