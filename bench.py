@@ -47,6 +47,11 @@ def parse():
     p.add_argument("--workload", default="cornell", choices=["cornell", "batch64", "guide_train"],
                    help="cornell: the headline frame; batch64: 64 uneven random-sphere renders in one batch; "
                         "guide_train: Adam steps of the guiding network on 2^18 rows")
+    p.add_argument("--launch", default=None, metavar="CAP,BLOCKS,GRAPH",
+                   help="batch64: use this launch (queue capacity, blocks per CU, graph iterations) instead of "
+                        "autotuning")
+    p.add_argument("--no-compare", action="store_true",
+                   help="batch64: skip the one-call-per-render and guided comparisons")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run one 256 MiB device copy first (the known-size kernel for HBM counter scaling)")
     return p.parse_args()
@@ -182,7 +187,11 @@ def batch64(tpt, torch, args):
 
     jobs, bufs = make_jobs(1)
     t0 = time.perf_counter()
-    launch, calib_ms = pool.autotune(jobs, calib_spp=1)
+    if args.launch:
+        launch, calib_ms = tpt.Launch(), None
+        launch.queue_capacity, launch.blocks_per_cu, launch.graph_iters = (int(x) for x in args.launch.split(","))
+    else:
+        launch, calib_ms = pool.autotune(jobs, calib_spp=1)
     t_tune = time.perf_counter() - t0
     samples = sum(sp["res"] ** 2 * sp["spp"] for sp in specs)
     for _ in range(args.warmup):
@@ -196,6 +205,36 @@ def batch64(tpt, torch, args):
         segs += st["segments"]
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    detail = {"Mrays_per_s": round(segs / args.steps / dt / 1e6, 2),
+              "launch": launch.as_dict(), "autotuned": not args.launch, "autotune_s": round(t_tune, 2),
+              "calibration_ms": None if calib_ms is None else round(calib_ms, 2),
+              "scene_build_s": round(t_build, 2)}
+    if not args.no_compare:
+        detail.update(batch64_compare(tpt, torch, pool, make_jobs, launch, dt, samples))
+    value = samples / dt / 1e6
+    return {
+        "metric": "Msamples/s (batched renders: 64 random-sphere scenes, uneven resolutions and spp, one batch)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic (64 procedural random-sphere scenes, seed {BATCH_SEED})",
+        "config": {"workload": "batch64: 64 random_spheres_n<N> scenes, N log-uniform in [50, 20000], "
+                               "res uniform in {128,256,512,1024}^2, spp uniform in {4,16,64,256}, depth 50",
+                   "samples": samples, "scenes": [sp["spheres"] for sp in specs]},
+        "detail": detail,
+    }
+
+
+def batch64_compare(tpt, torch, pool, make_jobs, launch, dt, samples):
+    """The batch's renders one tpt_render_jobs call each, and the batch with
+    the guide on."""
     # the same renders one call each (one job per tpt_render_jobs call, the
     # tpt_render path; default launch settings)
     jobs, bufs = make_jobs(3000)
@@ -227,32 +266,11 @@ def batch64(tpt, torch, args):
     pool.render_jobs(jobs, launch=gl, guide=g)
     torch.cuda.synchronize()
     dt_guided = time.perf_counter() - t0
-    value = samples / dt / 1e6
-    return {
-        "metric": "Msamples/s (batched renders: 64 random-sphere scenes, uneven resolutions and spp, one batch)",
-        "value": round(value, 3),
-        "unit": "Msamples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": f"synthetic (64 procedural random-sphere scenes, seed {BATCH_SEED})",
-        "config": {"workload": "batch64: 64 random_spheres_n<N> scenes, N log-uniform in [50, 20000], "
-                               "res uniform in {128,256,512,1024}^2, spp uniform in {4,16,64,256}, depth 50",
-                   "samples": samples, "scenes": [sp["spheres"] for sp in specs]},
-        "detail": {"Mrays_per_s": round(segs / args.steps / dt / 1e6, 2),
-                   "autotuned_launch": launch.as_dict(), "autotune_s": round(t_tune, 2),
-                   "calibration_ms": round(calib_ms, 2), "scene_build_s": round(t_build, 2),
-                   "one_call_per_render_ms": round(dt_single * 1e3, 2),
-                   "batch_speedup_vs_one_call_per_render": round(dt_single / dt, 3),
-                   "guided_batch_ms": round(dt_guided * 1e3, 2),
-                   "guided_Msamples_per_s": round(samples / dt_guided / 1e6, 3),
-                   "guide_training_s": round(t_train, 3)},
-    }
+    return {"one_call_per_render_ms": round(dt_single * 1e3, 2),
+            "batch_speedup_vs_one_call_per_render": round(dt_single / dt, 3),
+            "guided_batch_ms": round(dt_guided * 1e3, 2),
+            "guided_Msamples_per_s": round(samples / dt_guided / 1e6, 3),
+            "guide_training_s": round(t_train, 3)}
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0

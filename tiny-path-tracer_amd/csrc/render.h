@@ -348,14 +348,17 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
     store_ray(Q.rq[0], (w % NSUB) * Q.region + w / NSUB, raygen_item<MULTI>(P, H, w));
 }
 
-// SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  The
-// slot/entry arrays come in as separate restrict arguments so that
-// wave-uniform loads from them can use the scalar data path.
-template <bool SWEEP, bool MULTI>
-__global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
+// SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  SPH:
+// every leaf of the scene (or pool) is a sphere; without the rect and list
+// code the kernel fits 64 registers, 8 waves per SIMD.  The slot/entry arrays come
+// in as separate restrict arguments so that wave-uniform loads from them can
+// use the scalar data path.
+template <bool SWEEP, bool MULTI, bool SPH = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : 1))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
   static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");
+  static_assert(!(SWEEP && SPH), "the sweep takes every leaf kind");
   S.slots = slots;
   S.entries = entries;
   __shared__ uint32_t offs[NSUB];
@@ -374,27 +377,39 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < count;
     uint32_t cls = 3;
-    path p{};
+    // Without SWEEP only the ray and the (depth, entry) word stay live across
+    // the traversal (fewer registers, more waves); the third plane is read
+    // after it.  The sweep's waves are few and short of latency cover, so it
+    // reads the whole record first.
+    uint32_t src = 0;
+    float4 a{}, b{}, c0{};
     hit_t h{0.0f, kNone, kNone, 0};
     if (valid) {
       const uint32_t r = find_region(pre, NSUB, qi);
-      p = load_ray(in, r * Q.region + (qi - pre[r]));
+      src = r * Q.region + (qi - pre[r]);
+      a = in[0][src];
+      b = in[1][src];
+      if (SWEEP) c0 = in[2][src];
     }
-    const uint32_t job = MULTI ? P.pass[p.ent].job : 0u;
+    const uint32_t depth = __float_as_uint(b.w) & 0xFFFFu;
+    const uint32_t job = MULTI ? P.pass[__float_as_uint(b.w) >> 16].job : 0u;
     const scene_view Sj = view_of<MULTI>(S, P, job);
-    const bool got = trace_world<SWEEP>(Sj, p.o, p.d, 0.001f, kFltMax, &h, &st[ST_NODES], valid);
+    const bool got = trace_world<SWEEP, SPH>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, 0.001f, kFltMax, &h,
+                                             &st[ST_NODES], valid);
+    const float4 c = SWEEP || !valid ? c0 : in[2][src];
     if (valid) {
       ++st[ST_SEGMENTS];
       if (got) {
         h.mat &= TPT_PAYLOAD_MASK;
         const uint32_t kind = S.materials[h.mat].kind;
-        if (kind == TPT_MAT_DIFFUSE_LIGHT || p.depth >= P.jobs[job].max_depth) {
+        if (kind == TPT_MAT_DIFFUSE_LIGHT || depth >= P.jobs[job].max_depth) {
           cls = Q_TERMINAL;
         } else {
           // In guided mode only the first guide_verts diffuse vertices of a
           // path go to the diffuse queue (where the network runs); later
           // ones are shaded with the plain sampler from the other queue.
-          cls = kind == TPT_MAT_LAMBERTIAN && p.ndiff < P.guide_verts ? Q_DIFFUSE : Q_OTHER;
+          const uint32_t ndiff = __float_as_uint(c.w) & 0xFFFFu;
+          cls = kind == TPT_MAT_LAMBERTIAN && ndiff < P.guide_verts ? Q_DIFFUSE : Q_OTHER;
         }
       } else {
         h.slot = kNone;
@@ -405,7 +420,9 @@ __global__ void __launch_bounds__(256) k_trace(scene_view S, const tpt_slot *__r
     for (uint32_t k = 0; k < 3; ++k) {
       const uint32_t at = home * Q.region + wave_append(Q.counters + mc_at(parity, k, home), cls == k);
       if (cls == k) {
-        store_ray(Q.mq[k], at, p);
+        Q.mq[k][0][at] = a;
+        Q.mq[k][1][at] = b;
+        Q.mq[k][2][at] = c;
         Q.mq[k][3][at] = make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
       }
     }

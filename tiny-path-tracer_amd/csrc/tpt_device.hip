@@ -103,6 +103,7 @@ struct tpt_scene {
   // the pool's scenes
   std::vector<scene_dev> scn;
   std::vector<uint8_t> sweep;     // per scene: wave-uniform traversal (small scenes)
+  std::vector<uint8_t> sph;       // per scene: every leaf is a sphere
   std::vector<uint32_t> n_slots_of;
   std::vector<uint32_t> n_leaves_of;
   scene_dev *d_scenes = nullptr;
@@ -329,7 +330,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   std::vector<tpt_perlin> perlin;
   std::vector<tpt_light> lights;
   std::vector<scene_dev> scn(n);
-  std::vector<uint8_t> sweep(n);
+  std::vector<uint8_t> sweep(n), sph(n);
   std::vector<uint32_t> nslots(n), nleaves(n);
   for (uint32_t k = 0; k < n; ++k) {
     const tpt_flat_scene &f = scenes[k];
@@ -339,9 +340,11 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     if (sb + f.n_slots >= TPT_NO_RIGHT || mb + f.n_materials > TPT_PAYLOAD_MASK || bb + f.n_texels > 0xFFFFFFFFull ||
         eb + f.n_entries > 0xFFFFFFF0ull)
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: pool too large for 32-bit indices");
+    sph[k] = 1;
     for (uint32_t i = 0; i < f.n_slots; ++i) {
       tpt_slot s = f.slots[i];
       const uint32_t kind = s.b >> 28;
+      if (kind != TPT_SLOT_BOX && kind != TPT_SLOT_SPHERE) sph[k] = 0;
       if (kind == TPT_SLOT_BOX) {
         s.a += static_cast<uint32_t>(sb);
         const uint32_t right = s.b & TPT_PAYLOAD_MASK;
@@ -415,6 +418,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   s->n_lights = static_cast<uint32_t>(lights.size());
   s->scn = scn;
   s->sweep = sweep;
+  s->sph = sph;
   s->n_slots_of = nslots;
   s->n_leaves_of = nleaves;
   if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
@@ -589,14 +593,19 @@ struct bounce_launch {
   render_state Q;
   uint32_t grid, guide_grid;
   bool sweep, multi, guided;
+  bool sph;  // every leaf of the launch's scenes is a sphere
 };
 
 void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
   if (B.sweep)
     hipLaunchKernelGGL((k_trace<true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+  else if (B.multi && B.sph)
+    hipLaunchKernelGGL((k_trace<false, true, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
   else if (B.multi)
     hipLaunchKernelGGL((k_trace<false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+  else if (B.sph)
+    hipLaunchKernelGGL((k_trace<false, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
   else
     hipLaunchKernelGGL((k_trace<false, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
 }
@@ -705,15 +714,17 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   if ((rc = ensure_state(s, cap, max_items, grid, guide_grid)) != TPT_OK) return rc;
 
   bounce_launch B;
+  std::memset(&B, 0, sizeof(B));  // the captured-graph key compares its bytes
   B.S = multi ? s->pool_view() : s->view(jobs[0].scene);
   B.Q = s->Q;
   B.grid = grid;
   B.guide_grid = guide_grid;
   B.sweep = !multi && s->sweep[jobs[0].scene];
+  B.sph = true;
+  for (uint32_t j = 0; j < n_jobs; ++j) B.sph = B.sph && s->sph[jobs[j].scene];
   B.multi = multi;
   B.guided = guide != nullptr;
   render_params &P = B.P;
-  P = render_params{};
   P.jobs = s->d_jobs;
   P.scenes = s->d_scenes;
   P.hdr = s->d_hdr;

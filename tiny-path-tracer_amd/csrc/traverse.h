@@ -85,6 +85,26 @@ __device__ __forceinline__ bool slab(const slot_t &s, v3 o, v3 inv, float tmin, 
   return !(tmin > tmax);
 }
 
+// The same test when t_min and t_max are not NaN.  The reference's updates
+// `t0 > tmin ? t0 : tmin` and `t1 < tmax ? t1 : tmax` then equal IEEE maxNum
+// and minNum: a NaN t0 or t1 leaves the bound unchanged in both forms, and the
+// bounds stay non-NaN; where the forms pick a zero of the other sign the value
+// still compares equal, so every later comparison and the result agree.  The
+// swap becomes a select on the ray's fixed direction signs.
+__device__ __forceinline__ bool slab_nn(const slot_t &s, v3 o, v3 inv, float tmin, float tmax) {
+  const float x0 = (s.f[0] - o.x) * inv.x, x1 = (s.f[3] - o.x) * inv.x;
+  const float y0 = (s.f[1] - o.y) * inv.y, y1 = (s.f[4] - o.y) * inv.y;
+  const float z0 = (s.f[2] - o.z) * inv.z, z1 = (s.f[5] - o.z) * inv.z;
+  const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
+  tmin = fmaxf(nx ? x1 : x0, tmin);
+  tmax = fminf(nx ? x0 : x1, tmax);
+  tmin = fmaxf(ny ? y1 : y0, tmin);
+  tmax = fminf(ny ? y0 : y1, tmax);
+  tmin = fmaxf(nz ? z1 : z0, tmin);
+  tmax = fminf(nz ? z0 : z1, tmax);
+  return !(tmin > tmax);
+}
+
 __device__ __forceinline__ bool sphere_t(float cx, float cy, float cz, float r, v3 o, v3 d, float tmin,
                                          float tmax, float *t) {
   const float ox = o.x - cx, oy = o.y - cy, oz = o.z - cz;
@@ -187,9 +207,11 @@ __device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 
   return found;
 }
 
+// SPH: the scene's leaves are all spheres (no rect or list code is compiled).
+template <bool SPH = false>
 __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d, float tmin,
                                        float tmax, float *t, uint32_t *entry, uint32_t *mat) {
-  if (kind == TPT_SLOT_SPHERE) {
+  if (SPH || kind == TPT_SLOT_SPHERE) {
     *entry = kNone;
     *mat = s.a;
     return sphere_t(s.f[0], s.f[1], s.f[2], s.f[3], o, d, tmin, tmax, t);
@@ -205,6 +227,7 @@ __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uin
 // Exact replay of the bvh_node recursion for one segment: evaluates each node
 // with a fresh result, combining (left, right) as `left.t < right.t ? left :
 // right`.  Stack entries: a pending right child, or a finished left result.
+template <bool SPH = false>
 __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
                                                         float tmin, float tmax, hit_t *out) {
   struct frame {
@@ -233,7 +256,7 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
       } else {
         float t;
         uint32_t entry, mat;
-        have = leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat);
+        have = leaf_t<SPH>(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat);
         acc = hit_t{t, i, entry, mat};
       }
       break;
@@ -264,8 +287,9 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
   return have;
 }
 
-// Fast pre-order walk of one segment.  *saw_nan is set when a leaf returned a
-// NaN t (the caller then uses segment_exact).
+// Fast pre-order walk of one segment (t_min, t_max not NaN).  *saw_nan is set
+// when a leaf returned a NaN t (the caller then uses segment_exact).
+template <bool SPH = false>
 __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
                                              float tmin, float tmax, hit_t *out, bool *saw_nan, uint32_t *nodes) {
   bool have = false;
@@ -276,20 +300,21 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
     const slot_t s = load_slot(&S.slots[i]);
     const uint32_t kind = s.b >> 28;
     ++visited;
+    uint32_t ni = i + 1;
     if (kind == TPT_SLOT_BOX) {
-      i = slab(s, o, inv, tmin, tmax) ? i + 1 : s.a;
-      continue;
-    }
-    float t;
-    uint32_t entry, mat;
-    if (leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
-      if (t != t) *saw_nan = true;
-      if (!have || t <= best.t) {
-        best = hit_t{t, i, entry, mat};
-        have = true;
+      if (!slab_nn(s, o, inv, tmin, tmax)) ni = s.a;
+    } else {
+      float t;
+      uint32_t entry, mat;
+      if (leaf_t<SPH>(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
+        if (t != t) *saw_nan = true;
+        if (!have || t <= best.t) {
+          best = hit_t{t, i, entry, mat};
+          have = true;
+        }
       }
     }
-    ++i;
+    i = ni;
   }
   *nodes += visited;
   if (have) *out = best;
@@ -329,7 +354,7 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
     if (mine) {
       ++visited;
       if (kind == TPT_SLOT_BOX) {
-        next = slab(s, o, inv, tmin, tmax) ? i + 1 : s.a;
+        next = slab_nn(s, o, inv, tmin, tmax) ? i + 1 : s.a;
       } else {
         float t;
         uint32_t entry, mat;
@@ -354,8 +379,11 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
 }
 
 // World = segments evaluated like a hitable_list (t_max shrinks between them).
-// SWEEP selects segment_sweep (all lanes of the wave must call together).
-template <bool SWEEP = false>
+// SWEEP selects segment_sweep (all lanes of the wave must call together).  A
+// NaN bound (a caller's t_min, or t_max after a segment whose winner had a
+// NaN t) sends the segment to segment_exact, which keeps the reference's slab
+// updates verbatim.
+template <bool SWEEP = false, bool SPH = false>
 __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float tmin, float tmax, hit_t *out,
                                             uint32_t *nodes, bool live = true) {
   const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
@@ -366,14 +394,15 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, flo
     hit_t h;
     bool saw_nan = false;
     if (start >= end) continue;
-    bool got;
+    const bool bounded = tmin == tmin && cur == cur;
+    bool got = false;
     if (SWEEP) {
-      got = segment_sweep(S, start, end, live, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+      got = segment_sweep(S, start, end, live && bounded, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
     } else {
       if (!live) continue;
-      got = segment_fast(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+      if (bounded) got = segment_fast<SPH>(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
     }
-    if (live && saw_nan) got = segment_exact(S, start, o, d, inv, tmin, cur, &h);
+    if (live && (saw_nan || !bounded)) got = segment_exact<SPH>(S, start, o, d, inv, tmin, cur, &h);
     if (got) {
       *out = h;
       cur = h.t;
