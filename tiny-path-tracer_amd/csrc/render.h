@@ -498,9 +498,12 @@ __device__ __forceinline__ void claim_and_append(const render_params &P, const p
 
 // Shading of the material queues.  Plain mode: all three classes.  Guided
 // mode: the other and terminal classes; the diffuse class (the vertices that
-// use the guide) is shaded by k_guide_shade.
-template <bool GUIDED, bool MULTI>
-__global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+// use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures and
+// at most one rect light (see shade.h).  The occupancy targets below make the
+// register allocator fit 80 or 96 registers (6 or 5 waves per SIMD instead of
+// 4) without spilling.
+template <bool GUIDED, bool MULTI, bool SIMPLE = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE ? (MULTI ? 5 : 6) : (GUIDED ? 1 : 5)))) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
   // material regions in class-major order: region r = (class - C0) * NSUB + k
   constexpr uint32_t C0 = GUIDED ? 1u : 0u, NR = (3u - C0) * NSUB;
@@ -549,7 +552,7 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           // diffuse_light::emitted is one-sided (src/material.cc:79-86);
           // every other material emits nothing.
           v3 e{0.0f, 0.0f, 0.0f};
-          if (M.kind == TPT_MAT_DIFFUSE_LIGHT && dot(n, p.d) < 0.0f) e = tex_value(S, M.tex, u, v, pt);
+          if (M.kind == TPT_MAT_DIFFUSE_LIGHT && dot(n, p.d) < 0.0f) e = tex_value<SIMPLE>(S, M.tex, u, v, pt);
           p.rad = p.rad + p.thr * e;
         } else if (M.kind == TPT_MAT_METAL) {
           // metal::scatter (src/material.cc:88-98); specular: no emitted term.
@@ -585,16 +588,16 @@ __global__ void __launch_bounds__(256) k_shade(scene_view S, render_params P, re
           // lambertian + mixture_pdf(hitable_pdf(lights), cosine_pdf)
           // (src/material.cc:3-17, src/utils.cc:73-81); cosine_pdf::value
           // is cos θ without 1/π, as in the reference.
-          const v3 atten = tex_value(S, M.tex, u, v, pt);
+          const v3 atten = tex_value<SIMPLE>(S, M.tex, u, v, pt);
           const onb uvw(n);
           float pdf;
           if (Sj.n_lights > 0) {
             if (R.next() < 0.5f) {
-              nd = light_random(Sj, pt, R);
+              nd = light_random<SIMPLE>(Sj, pt, R);
             } else {
               nd = uvw.local(on_hemisphere(R));
             }
-            const float pl = light_pdf(Sj, pt, nd);
+            const float pl = light_pdf<SIMPLE>(Sj, pt, nd);
             ++st[ST_LIGHT];
             float pc = dot(unit(nd), uvw.w);
             pc = pc > 0.0f ? pc : 0.0f;

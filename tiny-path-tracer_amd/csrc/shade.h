@@ -125,12 +125,18 @@ __device__ float turb(const tpt_perlin *P, v3 p, uint32_t depth) {
   return fabsf(acc);
 }
 
+// SIMPLE (template flag of the shading code): every texture of the scene is a
+// constant_texture and the light root is absent or one xz_rect, so the
+// Perlin, image, checker, sphere-light and light-list code is left out
+// (fewer registers, more waves).
+
 // texture::value for the four textures (headers/texture.h:15-24,
 // src/texture.cc:4-42).  Checker nesting is followed iteratively.
+template <bool SIMPLE = false>
 __device__ v3 tex_value(const scene_view &S, uint32_t tex, float u, float v, v3 p) {
   for (int guard = 0; guard < 32; ++guard) {
     const tpt_texture T = S.textures[tex];
-    if (T.kind == TPT_TEX_CONSTANT) return v3{T.p[0], T.p[1], T.p[2]};
+    if (SIMPLE || T.kind == TPT_TEX_CONSTANT) return v3{T.p[0], T.p[1], T.p[2]};
     if (T.kind == TPT_TEX_CHECKER) {
       const float s = sinf(10.0f * p.x) * sinf(10.0f * p.y) * sinf(10.0f * p.z);
       tex = s < 0.0f ? T.a : T.b;
@@ -156,8 +162,9 @@ __device__ v3 tex_value(const scene_view &S, uint32_t tex, float u, float v, v3 
 // ------------------------------------------------------------ light list ---
 // pdf_value of one light (src/rect_box.cc:26-37, src/sphere.cc:93-106,
 // headers/hitable.h:35-37 for other hitables).
+template <bool SIMPLE = false>
 __device__ __forceinline__ float one_light_pdf(const tpt_light &L, v3 o, v3 d) {
-  if (L.kind == TPT_LIGHT_XZ_RECT) {
+  if (SIMPLE || L.kind == TPT_LIGHT_XZ_RECT) {
     const float f[5] = {L.p[4], L.p[0], L.p[1], L.p[2], L.p[3]};
     float t;
     if (rect_t(f, 1, o, d, 0.0001f, kFltMax, &t)) {
@@ -187,8 +194,9 @@ __device__ __forceinline__ float one_light_pdf(const tpt_light &L, v3 o, v3 d) {
 
 // hitable_pdf::value on the light root (src/utils.cc:452-454): a bare shape's
 // pdf, or hitable_list's weighted sum (src/hitable_list.cc:24-31).
+template <bool SIMPLE = false>
 __device__ __forceinline__ float light_pdf(const scene_view &S, v3 o, v3 d) {
-  if (S.light_mode == 0) return one_light_pdf(S.lights[0], o, d);
+  if (SIMPLE || S.light_mode == 0) return one_light_pdf<SIMPLE>(S.lights[0], o, d);
   float sum = 0.0f;
   for (uint32_t i = 0; i < S.n_lights; ++i) {
     const tpt_light L = S.lights[i];
@@ -200,9 +208,11 @@ __device__ __forceinline__ float light_pdf(const scene_view &S, v3 o, v3 d) {
 // hitable_pdf::generate (src/utils.cc:456): pick (list: int(u·n)), then the
 // shape's random(): a uniform point on the rect (src/rect_box.cc:39-43) or a
 // cone direction toward the sphere (src/sphere.cc:108-120).
+template <bool SIMPLE = false>
 __device__ __forceinline__ v3 light_random(const scene_view &S, v3 o, stream &R) {
   uint32_t idx = 0;
-  if (S.light_mode == 1) {
+  if (SIMPLE) {
+  } else if (S.light_mode == 1) {
     idx = static_cast<uint32_t>(static_cast<int>(static_cast<double>(R.next()) * S.n_lights));
     if (idx >= S.n_lights) idx = S.n_lights - 1;
   } else if (S.light_mode == 2) {
@@ -218,7 +228,7 @@ __device__ __forceinline__ v3 light_random(const scene_view &S, v3 o, stream &R)
     }
   }
   const tpt_light L = S.lights[idx];
-  if (L.kind == TPT_LIGHT_XZ_RECT) {
+  if (SIMPLE || L.kind == TPT_LIGHT_XZ_RECT) {
     const float a = R.next();
     const float b = R.next();
     const v3 pt{L.p[0] + a * (L.p[1] - L.p[0]), L.p[4], L.p[2] + b * (L.p[3] - L.p[2])};

@@ -104,6 +104,7 @@ struct tpt_scene {
   std::vector<scene_dev> scn;
   std::vector<uint8_t> sweep;     // per scene: wave-uniform traversal (small scenes)
   std::vector<uint8_t> sph;       // per scene: every leaf is a sphere
+  std::vector<uint8_t> simple;    // per scene: constant textures, no light or one xz_rect light
   std::vector<uint32_t> n_slots_of;
   std::vector<uint32_t> n_leaves_of;
   scene_dev *d_scenes = nullptr;
@@ -330,7 +331,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   std::vector<tpt_perlin> perlin;
   std::vector<tpt_light> lights;
   std::vector<scene_dev> scn(n);
-  std::vector<uint8_t> sweep(n), sph(n);
+  std::vector<uint8_t> sweep(n), sph(n), simple(n);
   std::vector<uint32_t> nslots(n), nleaves(n);
   for (uint32_t k = 0; k < n; ++k) {
     const tpt_flat_scene &f = scenes[k];
@@ -390,6 +391,8 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
     // close to each lane's own count only when the whole tree is small.
     sweep[k] = f.n_slots <= 64;
+    simple[k] = f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT);
+    for (uint32_t t = 0; t < f.n_textures; ++t) simple[k] = simple[k] && f.textures[t].kind == TPT_TEX_CONSTANT;
     nslots[k] = f.n_slots;
     nleaves[k] = f.n_leaves;
   }
@@ -419,6 +422,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   s->scn = scn;
   s->sweep = sweep;
   s->sph = sph;
+  s->simple = simple;
   s->n_slots_of = nslots;
   s->n_leaves_of = nleaves;
   if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
@@ -593,7 +597,8 @@ struct bounce_launch {
   render_state Q;
   uint32_t grid, guide_grid;
   bool sweep, multi, guided;
-  bool sph;  // every leaf of the launch's scenes is a sphere
+  bool sph;     // every leaf of the launch's scenes is a sphere
+  bool simple;  // their textures are constant and their light root absent or one xz_rect
 };
 
 void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
@@ -620,8 +625,12 @@ void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
       hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
       hipLaunchKernelGGL((k_guide_shade<false>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
     }
+  } else if (B.multi && B.simple) {
+    hipLaunchKernelGGL((k_shade<false, true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
   } else if (B.multi) {
     hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
+  } else if (B.simple) {
+    hipLaunchKernelGGL((k_shade<false, false, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
   } else {
     hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
   }
@@ -721,7 +730,11 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   B.guide_grid = guide_grid;
   B.sweep = !multi && s->sweep[jobs[0].scene];
   B.sph = true;
-  for (uint32_t j = 0; j < n_jobs; ++j) B.sph = B.sph && s->sph[jobs[j].scene];
+  B.simple = true;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    B.sph = B.sph && s->sph[jobs[j].scene];
+    B.simple = B.simple && s->simple[jobs[j].scene];
+  }
   B.multi = multi;
   B.guided = guide != nullptr;
   render_params &P = B.P;
