@@ -60,6 +60,7 @@ enum tpt_status {
 #define TPT_SLOT_SPHERE 1u
 #define TPT_SLOT_RECT 2u
 #define TPT_SLOT_LIST 3u
+#define TPT_SLOT_MSPHERE 4u
 #define TPT_PAYLOAD_MASK 0x0FFFFFFFu
 #define TPT_NO_RIGHT 0x0FFFFFFFu
 
@@ -73,6 +74,8 @@ enum tpt_status {
  *  RECT:   f[0] k, f[1] a0, f[2] a1, f[3] b0, f[4] b1 (in-plane bounds, in
  *          the order of the reference constructor); a = material | axis<<28 |
  *          flip<<30; b payload = leaf id.
+ *  MSPHERE: a moving_sphere (headers/sphere.h:23-40): f[0] = index into
+ *          motions (as uint32 bits); a = material; b payload = leaf id.
  *  LIST:   a = first entry; f[0] = leaf id of the first entry (as uint32 bits),
  *          f[1] = entry count (as uint32 bits).  A leaf made of wrappers
  *          (translate / rotate_y / flip_normal), box and hitable_list around
@@ -88,9 +91,10 @@ typedef struct tpt_slot {
 
 /* 32-byte entry of a LIST leaf: one primitive seen through a wrapper chain.
  *  f[0..4] primitive parameters as in tpt_slot (sphere: centre, radius;
- *  rect: k, a0, a1, b0, b1); xform = index into xforms or 0xFFFFFFFF;
+ *  rect: k, a0, a1, b0, b1; moving sphere: f[0] = motion index bits);
+ *  xform = index into xforms or 0xFFFFFFFF;
  *  a = material | axis<<28 | flip<<30 (flip = parity of flip_normal wrappers);
- *  b = TPT_SLOT_SPHERE or TPT_SLOT_RECT. */
+ *  b = TPT_SLOT_SPHERE, TPT_SLOT_RECT or TPT_SLOT_MSPHERE. */
 typedef struct tpt_entry {
   float f[5];
   uint32_t xform;
@@ -152,6 +156,18 @@ typedef struct tpt_light {
   float p[6];
 } tpt_light;
 
+/* moving_sphere parameters (headers/sphere.h:23-40, src/sphere.cc:3-11):
+ * centre(time) = center0 + (time - time0) / (time1 - time0) * (center1 - center0)
+ * in float, as moving_sphere::center computes it. */
+typedef struct tpt_motion {
+  float center0[3];
+  float radius;
+  float center1[3];
+  float time0;
+  float time1;
+  float pad[3];
+} tpt_motion;
+
 typedef struct tpt_flat_scene {
   const tpt_slot *slots;
   uint32_t n_slots;
@@ -176,6 +192,8 @@ typedef struct tpt_flat_scene {
   uint32_t n_leaves; /* number of leaf ids */
   uint32_t max_depth; /* deepest bvh_node nesting */
   float bounds[6];    /* world bounding_box (min xyz, max xyz); normalises guiding inputs */
+  const tpt_motion *motions; /* moving spheres (MSPHERE slots and entries index it) */
+  uint32_t n_motions;
 } tpt_flat_scene;
 
 /* Camera fields as computed by camera_with_blur's constructor
@@ -211,6 +229,7 @@ typedef struct tpt_scene_info {
   uint32_t n_leaves;
   float lookfrom[3], lookat[3];           /* its camera: vup (0,1,0), vfov degrees, aperture, focus */
   float vfov, aperture, focus;
+  float time0, time1;                     /* its shutter */
 } tpt_scene_info;
 
 const char *tpt_scene_names(void);
@@ -256,6 +275,13 @@ int tpt_trace_closest(tpt_scene *scene, const float *d_rays, uint64_t n, float t
 int tpt_trace_closest_record(tpt_scene *scene, const float *d_rays, uint64_t n, float t_min,
                              float t_max, int32_t *d_leaf, float *d_t, float *d_rec,
                              int32_t *d_mat, void *stream);
+
+/* Rays with a time (ray::time(), headers/ray.h): d_time = n floats, the time
+ * each ray's moving spheres are placed at (NULL: time 0, the default of the
+ * reference's ray constructor).  d_rec and d_mat may be NULL. */
+int tpt_trace_closest_timed(tpt_scene *scene, const float *d_rays, const float *d_time, uint64_t n,
+                            float t_min, float t_max, int32_t *d_leaf, float *d_t, float *d_rec,
+                            int32_t *d_mat, void *stream);
 
 /* ----------------------------------------------------------------- render ---
  * Replaces color() (src/utils.cc:58-94) driven by the sample loop of

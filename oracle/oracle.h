@@ -6,9 +6,9 @@
 // tiny-path-tracer), evaluated directly on the object graph built by the
 // scene API (tiny-path-tracer_amd/scene), by recursive virtual-style
 // dispatch, with the reference's exact float/double arithmetic:
-//   hit()      bvh_node / AABB / hitable_list / sphere / rects / flip_normal /
-//              box / translate / rotate_y  (src/hitable.cc:63-90, src/aabb.cc:3-19,
-//              src/hitable_list.cc:38-51, src/sphere.cc:13-43,
+//   hit()      bvh_node / AABB / hitable_list / sphere / moving_sphere / rects /
+//              flip_normal / box / translate / rotate_y  (src/hitable.cc:63-90,
+//              src/aabb.cc:3-19, src/hitable_list.cc:38-51, src/sphere.cc:13-75,
 //              src/rect_box.cc:8-24,51-68,75-91,117-119,171-195,
 //              headers/rect_box.h:50-57,87-95)
 //   color()    src/utils.cc:58-94 with materials (src/material.cc), pdfs

@@ -150,11 +150,13 @@ int oracle_camera_fields(const oracle_scene *s, float *out) {
 }
 
 // rays: 6 arrays of n floats (ox, oy, oz, dx, dy, dz).  rec: 8 arrays of n
-// (u, v, px, py, pz, nx, ny, nz); rec and mat may be null.
+// (u, v, px, py, pz, nx, ny, nz); rec and mat may be null.  time: n ray times
+// (ray::time()), or null for time 0.
 int oracle_trace(const oracle_scene *s, const float *rays, uint64_t n, float tmin, float tmax, int32_t *leaf,
-                 float *t, float *rec, int32_t *mat, int nthreads) {
+                 float *t, float *rec, int32_t *mat, int nthreads, const float *time) {
   parallel_for(n, nthreads, [&](uint64_t i) {
-    const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]), vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+    const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]), vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]),
+                time ? time[i] : 0.0f);
     hit_record hr;
     uint32_t lf = 0;
     if (oracle::hit(s->spec.world, s->g, r, tmin, tmax, hr, 0, &lf)) {

@@ -54,6 +54,7 @@ void number_materials(const hitable *h, graph_info *g) {
   } else {
     const material *m = nullptr;
     if (auto *s = dynamic_cast<const sphere *>(h)) m = s->mat_ptr_;
+    if (auto *ms = dynamic_cast<const moving_sphere *>(h)) m = ms->mat_ptr_;
     if (auto *r1 = dynamic_cast<const xy_rect *>(h)) m = r1->mat_ptr_;
     if (auto *r2 = dynamic_cast<const xz_rect *>(h)) m = r2->mat_ptr_;
     if (auto *r3 = dynamic_cast<const yz_rect *>(h)) m = r3->mat_ptr_;
@@ -91,6 +92,33 @@ bool sphere_hit(const sphere &s, const ray &r, float t_min, float t_max, hit_rec
         rec.point = r.point_at_parameter(temp);
         uv_map((rec.point - s.center_) / s.radius_, rec.u, rec.v);
         rec.normal = (rec.point - s.center_) / s.radius_;
+        rec.mat_ptr = s.mat_ptr_;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+// src/sphere.cc:45-75: sphere::hit's arithmetic about center(r.time()); the
+// normal uses that centre, the uv the centre at time0.
+bool moving_sphere_hit(const moving_sphere &s, const ray &r, float t_min, float t_max, hit_record &rec) {
+  const vec3 oc = r.origin() - s.center(r.time());
+  const float a = dot(r.direction(), r.direction());
+  const float b = static_cast<float>(2.0 * static_cast<double>(dot(r.direction(), oc)));
+  const float c = dot(oc, oc) - s.radius_ * s.radius_;
+  const float disc = b * b - 4 * a * c;
+  if (disc > 0) {
+    for (int root = 0; root < 2; ++root) {
+      const float d2 = b * b - 4 * a * c;
+      const double sq = ::sqrt(static_cast<double>(d2));
+      const double num = root == 0 ? (static_cast<double>(-b) - sq) : (static_cast<double>(-b) + sq);
+      const float temp = static_cast<float>(num / static_cast<double>(2 * a));
+      if (temp < t_max && temp > t_min) {
+        rec.t = temp;
+        rec.point = r.point_at_parameter(temp);
+        rec.normal = (rec.point - s.center(r.time())) / s.radius_;
+        uv_map((rec.point - s.center0_) / s.radius_, rec.u, rec.v);
         rec.mat_ptr = s.mat_ptr_;
         return true;
       }
@@ -222,6 +250,11 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
   }
   if (auto *sp = dynamic_cast<const sphere *>(h)) {
     if (!sphere_hit(*sp, r, t_min, t_max, rec)) return false;
+    *leaf = base;
+    return true;
+  }
+  if (auto *ms = dynamic_cast<const moving_sphere *>(h)) {
+    if (!moving_sphere_hit(*ms, r, t_min, t_max, rec)) return false;
     *leaf = base;
     return true;
   }

@@ -84,6 +84,8 @@ struct tagger {
       walk(r->ptr_);
     } else if (auto *s = dynamic_cast<sphere *>(h)) {
       tag(&s->mat_ptr_);
+    } else if (auto *ms = dynamic_cast<moving_sphere *>(h)) {
+      tag(&ms->mat_ptr_);
     } else if (auto *q = dynamic_cast<xy_rect *>(h)) {
       tag(&q->mat_ptr_);
     } else if (auto *q2 = dynamic_cast<xz_rect *>(h)) {
@@ -135,6 +137,8 @@ bool build(const std::string &name, ref_scene_t *s, hitable **world) {
     *world = tpt_scene_ref_two_checker_spheres();
   } else if (name == "box_list") {
     *world = tpt_scene_box_list();
+  } else if (name == "ref_random_scene") {
+    *world = tpt_scene_ref_random_scene();
   } else if (name == "native:cornell_box") {
     // The reference's own factory and its driver's light list (main.cpp:99-106).
     *world = cornell_box();
@@ -212,7 +216,7 @@ int ref_leaf_count(void *sp) {
 }
 
 int ref_trace(void *sp, const float *rays, uint64_t n, float tmin, float tmax, int32_t *leaf, float *t, float *rec,
-              int32_t *mat, int nthreads) {
+              int32_t *mat, int nthreads, const float *time) {
   auto *s = static_cast<ref_scene_t *>(sp);
   std::atomic<uint64_t> next{0};
   auto worker = [&]() {
@@ -222,7 +226,7 @@ int ref_trace(void *sp, const float *rays, uint64_t n, float tmin, float tmax, i
       const uint64_t e = std::min<uint64_t>(n, b + 1024);
       for (uint64_t i = b; i < e; ++i) {
         const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]),
-                    vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+                    vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]), time ? time[i] : 0.0f);
         hit_record hr;
         if (s->world->hit(r, tmin, tmax, hr)) {
           auto *p = static_cast<proxy_material *>(hr.mat_ptr);

@@ -40,7 +40,7 @@ class Oracle:
         L.oracle_scene_settings.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4 + [C.POINTER(C.c_float)]
         L.oracle_camera_fields.argtypes = [C.c_void_p, _f32p]
         L.oracle_trace.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_float, C.c_float, _i32p, _f32p,
-                                   C.c_void_p, C.c_void_p, C.c_int]
+                                   C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.oracle_make_rays.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, _f32p, C.c_int]
         L.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                     C.c_uint32, C.c_float, C.c_int, C.c_int, _f32p, C.c_void_p]
@@ -116,16 +116,17 @@ class OracleScene:
         self.lib.L.oracle_make_rays(self.h, kind, n, seed, rays.reshape(-1), nthreads or _nthreads())
         return rays
 
-    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None):
+    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None, time=None):
         n = rays.shape[1]
         rays = np.ascontiguousarray(rays, np.float32)
         leaf = np.zeros(n, np.int32)
         t = np.zeros(n, np.float32)
         rec = np.zeros((8, n), np.float32) if record else None
         mat = np.zeros(n, np.int32) if record else None
+        time = None if time is None else np.ascontiguousarray(time, np.float32)
         self.lib.L.oracle_trace(self.h, rays.reshape(-1), n, tmin, tmax, leaf, t,
                                 rec.ctypes.data if record else None, mat.ctypes.data if record else None,
-                                nthreads or _nthreads())
+                                nthreads or _nthreads(), None if time is None else time.ctypes.data)
         return (leaf, t, rec, mat) if record else (leaf, t)
 
     def render(self, width, height, spp, max_depth=None, seed=1, sample_offset=0, background=None,
@@ -149,7 +150,7 @@ class Reference:
         L.ref_scene_build.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
         L.ref_leaf_count.argtypes = [C.c_void_p]
         L.ref_trace.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_float, C.c_float, _i32p, _f32p,
-                                C.c_void_p, C.c_void_p, C.c_int]
+                                C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.ref_camera_fields.argtypes = [_f32p, _f32p]
         L.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
         L.ref_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p,
@@ -164,15 +165,17 @@ class Reference:
     def leaf_count(self, h):
         return self.L.ref_leaf_count(h)
 
-    def trace(self, h, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None):
+    def trace(self, h, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, nthreads=None, time=None):
         n = rays.shape[1]
         rays = np.ascontiguousarray(rays, np.float32)
         leaf = np.zeros(n, np.int32)
         t = np.zeros(n, np.float32)
         rec = np.zeros((8, n), np.float32) if record else None
         mat = np.zeros(n, np.int32) if record else None
+        time = None if time is None else np.ascontiguousarray(time, np.float32)
         self.L.ref_trace(h, rays.reshape(-1), n, tmin, tmax, leaf, t, rec.ctypes.data if record else None,
-                         mat.ctypes.data if record else None, nthreads or _nthreads())
+                         mat.ctypes.data if record else None, nthreads or _nthreads(),
+                         None if time is None else time.ctypes.data)
         return (leaf, t, rec, mat) if record else (leaf, t)
 
     def camera_fields(self, params):

@@ -92,6 +92,30 @@ def test_camera_matches_oracle(tpt, oracle):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_moving_spheres_flatten(tpt, oracle):
+    # the reference's random_scene: its small lambertian / metal spheres are
+    # moving_spheres (MSPHERE slots indexing the motion table)
+    import ctypes as C
+
+    hs = tpt.HostScene("ref_random_scene")
+    f = hs.flat
+    assert f.n_motions > 100 and f.n_leaves == oracle.scene("ref_random_scene").n_leaves
+    slots = hs.slots()
+    kinds = np.array([b >> 28 for (_, _, b) in slots])
+    ms = [f_ for (f_, _, b) in slots if (b >> 28) == 4]
+    assert len(ms) == f.n_motions
+    idx = sorted(np.array([m[0] for m in ms], np.float32).view(np.uint32).tolist())
+    assert idx == list(range(f.n_motions))
+    # (center0, radius, center1, time0, time1, pad): r = 0.2 over the shutter [0, 1],
+    # both centres at the sphere grid's height band
+    mot = np.ctypeslib.as_array(C.cast(f.motions, C.POINTER(C.c_float)), shape=(f.n_motions, 12))
+    assert np.all(mot[:, 3] == np.float32(0.2)) and np.all(mot[:, 7] == 0) and np.all(mot[:, 8] == 1)
+    assert np.all(mot[:, 1] == np.float32(0.2)) and np.all(np.abs(mot[:, 5] - 0.2) <= 0.5 + 1e-6)
+    assert (kinds == 1).sum() > 0  # glass and the large spheres stay static
+    info = hs.info
+    assert info.time0 == 0.0 and info.time1 == 1.0
+
+
 def test_unsupported_types_fail_cleanly(tpt):
     # the product scene library has no unsupported scene; the flattener's
     # refusal is exercised through a scene name that does not exist

@@ -15,9 +15,10 @@ SMALL = ["ref_cornell_box", "box_list", "random_spheres_textured", "ref_light_sp
 CHUNK = 1 << 22
 
 
-def compare(s_or, sc, rays, torch, device, record=True):
-    want = s_or.trace(rays, record=record)
-    got = sc.trace(torch.from_numpy(rays).to(device), record=record)
+def compare(s_or, sc, rays, torch, device, record=True, time=None):
+    want = s_or.trace(rays, record=record, time=time)
+    got = sc.trace(torch.from_numpy(rays).to(device), record=record,
+                   time=None if time is None else torch.from_numpy(time).to(device))
     got = [g.cpu().numpy() for g in got]
     leaf_ok = np.array_equal(got[0], want[0])
     hit = want[0] >= 0
@@ -95,3 +96,21 @@ def test_trace_tmin_tmax_and_empty(tpt, oracle, device):
     empty = torch.zeros((6, 0), dtype=torch.float32, device=device)
     leaf, t = sc.trace(empty)
     assert leaf.numel() == 0
+
+
+def test_trace_moving_spheres(tpt, oracle, device):
+    """Rays with times (tpt_trace_closest_timed) on the reference's
+    random_scene: moving spheres placed at each ray's time."""
+    import torch
+
+    s = oracle.scene("ref_random_scene")
+    sc = tpt.Scene.named("ref_random_scene")
+    rng = np.random.default_rng(3)
+    for kind in range(4):
+        rays = s.make_rays(kind, 1 << 16, seed=177 + kind)
+        time = rng.uniform(0, 1, rays.shape[1]).astype(np.float32)
+        time[:8] = [0, 1, 0.5, 2, -1, 1e-7, 0.9999999, 3]
+        compare(s, sc, rays, torch, device, time=time)
+    # without times the rays see every sphere at time 0 (the ray default)
+    rays = s.make_rays(0, 1 << 14, seed=5)
+    compare(s, sc, rays, torch, device, record=False)

@@ -46,3 +46,29 @@ def test_oracle_color_matches_reference_color_statistically(oracle, reference):
     assert np.mean(np.abs(z) > 3) < 0.015
     se = np.sqrt((q_o.mean() + q_r.mean()) / spp / (w * w * 3 * spp))
     assert abs(m_o.mean() - m_r.mean()) <= 4 * se
+
+
+def test_moving_spheres_equal_reference(oracle, reference):
+    """moving_sphere::hit (src/sphere.cc:45-75) about center(r.time()): the
+    reference's random_scene with its moving spheres, rays at random times in
+    the shutter, at its ends and outside it (the centre extrapolates)."""
+    s = oracle.scene("ref_random_scene")
+    h = reference.scene("ref_random_scene")
+    assert s.n_leaves == reference.leaf_count(h)
+    rng = np.random.default_rng(7)
+    for kind in range(4):
+        rays = s.make_rays(kind, 1 << 13, seed=199 + kind)
+        n = rays.shape[1]
+        time = rng.uniform(0, 1, n).astype(np.float32)
+        time[:16] = [0, 1, 0.5, 2, -1, 0.25, 0.75, 1e-7, 0.9999999, 3, -0.5, 0, 1, 0.1, 0.2, 0.3]
+        a = s.trace(rays, record=True, time=time)
+        b = reference.trace(h, rays, record=True, time=time)
+        assert np.array_equal(a[0], b[0]), f"leaf ids differ (kind {kind})"
+        assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)), f"t differs (kind {kind})"
+        assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32)), f"record differs (kind {kind})"
+        assert np.array_equal(a[3], b[3])
+    # the hits do depend on the time: a moving sphere seen at t = 0 and t = 1
+    rays = s.make_rays(0, 1 << 13, seed=5)
+    l0 = s.trace(rays, time=np.zeros(rays.shape[1], np.float32))[0]
+    l1 = s.trace(rays, time=np.ones(rays.shape[1], np.float32))[0]
+    assert np.mean(l0 != l1) > 0.005

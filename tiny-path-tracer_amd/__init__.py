@@ -48,6 +48,7 @@ class FlatScene(C.Structure):
         ("perlin", C.c_void_p),
         ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("light_mode", C.c_uint32),
         ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("bounds", C.c_float * 6),
+        ("motions", C.c_void_p), ("n_motions", C.c_uint32),
     ]
 
 
@@ -69,7 +70,8 @@ class Camera(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
                 ("background", C.c_float), ("n_leaves", C.c_uint32), ("lookfrom", C.c_float * 3),
-                ("lookat", C.c_float * 3), ("vfov", C.c_float), ("aperture", C.c_float), ("focus", C.c_float)]
+                ("lookat", C.c_float * 3), ("vfov", C.c_float), ("aperture", C.c_float), ("focus", C.c_float),
+                ("time0", C.c_float), ("time1", C.c_float)]
 
 
 class RenderDesc(C.Structure):
@@ -132,6 +134,7 @@ def lib():
     L.tpt_scene_destroy.argtypes = [vp]
     L.tpt_trace_closest.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp]
     L.tpt_trace_closest_record.argtypes = [vp, vp, u64, f32, f32, i32p, vp, vp, i32p, vp]
+    L.tpt_trace_closest_timed.argtypes = [vp, vp, vp, u64, f32, f32, i32p, vp, vp, i32p, vp]
     L.tpt_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp, C.POINTER(RenderStats), vp]
     L.tpt_render_guided.argtypes = [vp, vp, C.POINTER(Camera), C.POINTER(RenderDesc), vp, vp,
                                     C.POINTER(RenderStats), vp]
@@ -274,8 +277,9 @@ class Scene:
         except Exception:
             pass
 
-    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, stream=None):
-        """rays: float32 tensor (6, n) on the device (ox, oy, oz, dx, dy, dz).
+    def trace(self, rays, tmin=0.001, tmax=3.4028234663852886e38, record=False, stream=None, time=None):
+        """rays: float32 tensor (6, n) on the device (ox, oy, oz, dx, dy, dz);
+        time: optional float32 tensor (n) of ray times (moving spheres).
         Returns (leaf int32[n], t float32[n]) [+ (rec float32[8, n], mat int32[n])]."""
         torch = _torch()
         assert rays.is_cuda and rays.dtype == torch.float32 and rays.shape[0] == 6
@@ -284,6 +288,16 @@ class Scene:
         leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
         t = torch.empty(n, dtype=torch.float32, device=rays.device)
         s = stream if stream is not None else torch.cuda.current_stream(rays.device).cuda_stream
+        if time is not None:
+            time = time.contiguous()
+            assert time.is_cuda and time.dtype == torch.float32 and time.shape == (n,)
+            rec = torch.empty((8, n), dtype=torch.float32, device=rays.device) if record else None
+            mat = torch.empty(n, dtype=torch.int32, device=rays.device) if record else None
+            _check(lib().tpt_trace_closest_timed(self.h, rays.data_ptr(), time.data_ptr(), n, tmin, tmax,
+                                                 leaf.data_ptr(), t.data_ptr(), rec.data_ptr() if record else None,
+                                                 mat.data_ptr() if record else None, s),
+                   "tpt_trace_closest_timed")
+            return (leaf, t, rec, mat) if record else (leaf, t)
         if record:
             rec = torch.empty((8, n), dtype=torch.float32, device=rays.device)
             mat = torch.empty(n, dtype=torch.int32, device=rays.device)

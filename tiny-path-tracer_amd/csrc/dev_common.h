@@ -47,6 +47,7 @@ struct scene_view {
   const uint8_t *texels;
   const tpt_perlin *perlin;
   const tpt_light *lights;
+  const tpt_motion *motions;
   uint32_t n_slots, n_segments, n_lights, light_mode;
 };
 

@@ -108,6 +108,7 @@ struct render_params {
   uint32_t rec_stride;      // record training rows from items with item % rec_stride == 0 (0: off)
   float4 *rec_side;         // per recorded path: 2 vertices x 5 float4
   float4 *rec_rows;         // training rows: 2 per recorded path, 4 float4 each
+  uint32_t motion;          // some scene has moving spheres: rays need their time
 };
 
 constexpr uint32_t REC_VERTS = 2;
@@ -307,7 +308,7 @@ __device__ __forceinline__ path raygen(const render_params &P, uint32_t item, co
   const v3 disk = in_unit_disk(R);
   const v3 rd = disk * C.lens_radius;
   const v3 off = cam_vec(C.u) * rd.x + cam_vec(C.v) * rd.y;
-  (void)R.next();  // ray time draw (time0 + xi*(time1 - time0)); static scenes ignore it
+  (void)R.next();  // the ray time draw; ray_time() replays the stream up to it
   const v3 org = cam_vec(C.origin);
   path p;
   p.o = org + off;
@@ -319,6 +320,23 @@ __device__ __forceinline__ path raygen(const render_params &P, uint32_t item, co
   p.ent = c.ent;
   p.ndiff = 0;
   return p;
+}
+
+// The time of the camera ray of item c (camera_with_blur::get_ray,
+// src/camera.cc:26: time0 + drand * (time1 - time0), in double as there).
+// Every ray of a path keeps it (ray(p, dir, r.time()) in color() and the
+// materials), so it is recomputed from the item's bounce-0 stream instead of
+// being stored in the path record; only scenes with moving spheres need it.
+__device__ __forceinline__ float ray_time(const render_params &P, const item_ctx &c) {
+  const job_dev &J = P.jobs[c.job];
+  stream R;
+  R.init(c.px, c.smp, 0, J.seed);
+  (void)R.next();
+  (void)R.next();
+  (void)in_unit_disk(R);
+  const float xi = R.next();
+  return static_cast<float>(static_cast<double>(J.cam.time0) +
+                            static_cast<double>(xi) * static_cast<double>(J.cam.time1 - J.cam.time0));
 }
 
 template <bool MULTI>
@@ -392,9 +410,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
       if (SWEEP) c0 = in[2][src];
     }
     const uint32_t depth = __float_as_uint(b.w) & 0xFFFFu;
-    const uint32_t job = MULTI ? P.pass[__float_as_uint(b.w) >> 16].job : 0u;
+    const uint32_t ent = MULTI ? __float_as_uint(b.w) >> 16 : 0u;
+    const uint32_t job = MULTI ? P.pass[ent].job : 0u;
     const scene_view Sj = view_of<MULTI>(S, P, job);
-    const bool got = trace_world<SWEEP, SPH>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, 0.001f, kFltMax, &h,
+    // (scenes with moving spheres use neither the sweep nor the sphere-only walk)
+    const float time =
+        !SWEEP && !SPH && P.motion && valid ? ray_time(P, item_in_entry(P, ent, __float_as_uint(a.w))) : 0.0f;
+    const bool got = trace_world<SWEEP, SPH>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, time, 0.001f, kFltMax, &h,
                                              &st[ST_NODES], valid);
     const float4 c = SWEEP || !valid ? c0 : in[2][src];
     if (valid) {
@@ -498,8 +520,8 @@ __device__ __forceinline__ void claim_and_append(const render_params &P, const p
 
 // Shading of the material queues.  Plain mode: all three classes.  Guided
 // mode: the other and terminal classes; the diffuse class (the vertices that
-// use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures and
-// at most one rect light (see shade.h).  The occupancy targets below make the
+// use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures, at
+// most one rect light (see shade.h) and no moving spheres.  The occupancy targets below make the
 // register allocator fit 80 or 96 registers (6 or 5 waves per SIMD instead of
 // 4) without spilling.
 template <bool GUIDED, bool MULTI, bool SIMPLE = false>
@@ -543,7 +565,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
       } else {
         v3 pt, n;
         float u, v;
-        hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+        hit_record(S, h, p.o, p.d, !SIMPLE && P.motion ? ray_time(P, c) : 0.0f, &pt, &n, &u, &v);
         const tpt_material M = S.materials[h.mat];
         stream R;
         R.init(c.px, c.smp, p.depth + 1, J.seed);
@@ -680,7 +702,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       v3 pt, n;
       float u, v;
-      hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+      const float time = P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f;
+      hit_record(S, h, p.o, p.d, time, &pt, &n, &u, &v);
       const v3 wo = -unit(p.d);
       const float pv[3] = {pt.x, pt.y, pt.z};
       const float *bounds = P.scenes[P.jobs[MULTI ? P.pass[p.ent].job : 0u].scene].bounds;
@@ -726,16 +749,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       p = load_ray(Q.mq[Q_DIFFUSE], off);
       const float4 hw = Q.mq[Q_DIFFUSE][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
+      const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
       v3 pt, n;
       float u, v;
-      hit_record(S, h, p.o, p.d, &pt, &n, &u, &v);
+      hit_record(S, h, p.o, p.d, P.motion ? ray_time(P, c) : 0.0f, &pt, &n, &u, &v);
       vmf_mix mix;
       const bool finite = decode_mix_bf16(reinterpret_cast<const uint16_t *>(outs + threadIdx.x * 10), &mix);
       mix_norms(&mix);
       // a non-finite network output falls back to the reference sampler at this vertex
       const float alpha = finite ? alpha_net : 0.0f;
       const tpt_material M = S.materials[h.mat];
-      const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
       const scene_view Sj = view_of<MULTI>(S, P, c.job);
       const bool lights = Sj.n_lights > 0;
       stream R;

@@ -22,7 +22,8 @@ namespace {
 template <bool SWEEP>
 __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_slot *__restrict__ slots,
                                                        const tpt_entry *__restrict__ entries,
-                                                       const float *__restrict__ rays, uint64_t n, float tmin,
+                                                       const float *__restrict__ rays,
+                                                       const float *__restrict__ times, uint64_t n, float tmin,
                                                        float tmax, int32_t *__restrict__ leaf,
                                                        float *__restrict__ tout, float *__restrict__ rec,
                                                        int32_t *__restrict__ mat) {
@@ -31,13 +32,15 @@ __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_s
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = i < n;
   v3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f};
+  float time = 0.0f;
   if (live) {
     o = v3{rays[i], rays[n + i], rays[2 * n + i]};
     d = v3{rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]};
+    if (times) time = times[i];
   }
   hit_t h;
   uint32_t nodes = 0;
-  const bool got = trace_world<SWEEP>(S, o, d, tmin, tmax, &h, &nodes, live);
+  const bool got = trace_world<SWEEP>(S, o, d, time, tmin, tmax, &h, &nodes, live);
   if (!live) return;
   if (got) {
     leaf[i] = static_cast<int32_t>(leaf_id(S, h));
@@ -45,7 +48,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_s
     if (rec) {
       v3 p, nn;
       float u, v;
-      hit_record(S, h, o, d, &p, &nn, &u, &v);
+      hit_record(S, h, o, d, time, &p, &nn, &u, &v);
       const float vals[8] = {u, v, p.x, p.y, p.z, nn.x, nn.y, nn.z};
 #pragma unroll
       for (int k = 0; k < 8; ++k) rec[k * n + i] = vals[k];
@@ -99,12 +102,14 @@ struct tpt_scene {
   uint8_t *texels = nullptr;
   tpt_perlin *perlin = nullptr;
   tpt_light *lights = nullptr;
+  tpt_motion *motions = nullptr;
   uint32_t n_slots = 0, n_segments = 0, n_lights = 0;
   // the pool's scenes
   std::vector<scene_dev> scn;
   std::vector<uint8_t> sweep;     // per scene: wave-uniform traversal (small scenes)
   std::vector<uint8_t> sph;       // per scene: every leaf is a sphere
   std::vector<uint8_t> simple;    // per scene: constant textures, no light or one xz_rect light
+  std::vector<uint8_t> motion;    // per scene: has moving spheres (rays need their time)
   std::vector<uint32_t> n_slots_of;
   std::vector<uint32_t> n_leaves_of;
   scene_dev *d_scenes = nullptr;
@@ -141,6 +146,7 @@ struct tpt_scene {
     v.texels = texels;
     v.perlin = perlin;
     v.lights = lights + scn[k].light_begin;
+    v.motions = motions;
     v.n_slots = n_slots;
     v.n_segments = scn[k].n_seg;
     v.n_lights = scn[k].n_lights;
@@ -170,6 +176,7 @@ void free_scene(tpt_scene *s) {
   (void)hipFree(s->texels);
   (void)hipFree(s->perlin);
   (void)hipFree(s->lights);
+  (void)hipFree(s->motions);
   (void)hipFree(s->d_scenes);
   (void)hipFree(s->state);
   (void)hipFree(s->d_jobs);
@@ -267,17 +274,24 @@ int validate_flat(const tpt_flat_scene *f) {
       std::memcpy(&cnt, &s.f[1], 4);
       if (static_cast<uint64_t>(s.a) + cnt > f->n_entries)
         return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: list entries out of range");
-    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT) {
+    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT || kind == TPT_SLOT_MSPHERE) {
       if ((s.a & TPT_PAYLOAD_MASK) >= f->n_materials)
         return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: material index out of range");
+      uint32_t mi;
+      std::memcpy(&mi, &s.f[0], 4);
+      if (kind == TPT_SLOT_MSPHERE && (!f->motions || mi >= f->n_motions))
+        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: motion index out of range");
     } else {
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: unknown slot kind");
     }
   }
   for (uint32_t e = 0; e < f->n_entries; ++e) {
     const tpt_entry &en = f->entries[e];
+    uint32_t mi;
+    std::memcpy(&mi, &en.f[0], 4);
     if ((en.a & TPT_PAYLOAD_MASK) >= f->n_materials || (en.xform != 0xFFFFFFFFu && en.xform >= f->n_xforms) ||
-        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT))
+        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT && en.b != TPT_SLOT_MSPHERE) ||
+        (en.b == TPT_SLOT_MSPHERE && (!f->motions || mi >= f->n_motions)))
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad list entry");
   }
   for (uint32_t x = 0; x < f->n_xforms; ++x)
@@ -314,6 +328,15 @@ uint32_t rebase_word(uint32_t word, uint32_t base) {
   return (word & ~TPT_PAYLOAD_MASK) | ((word & TPT_PAYLOAD_MASK) + base);
 }
 
+// An index kept as the bits of a float field (the motion index of MSPHERE).
+float rebase_bits(float f, uint32_t base) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += base;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
 int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   if (!scenes || !out || n == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: no scenes");
   *out = nullptr;
@@ -330,14 +353,15 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   std::vector<uint8_t> texels;
   std::vector<tpt_perlin> perlin;
   std::vector<tpt_light> lights;
+  std::vector<tpt_motion> motions;
   std::vector<scene_dev> scn(n);
-  std::vector<uint8_t> sweep(n), sph(n), simple(n);
+  std::vector<uint8_t> sweep(n), sph(n), simple(n), motion(n);
   std::vector<uint32_t> nslots(n), nleaves(n);
   for (uint32_t k = 0; k < n; ++k) {
     const tpt_flat_scene &f = scenes[k];
     const uint64_t sb = slots.size(), eb = entries.size(), xb = xforms.size(), mb = materials.size(),
                    tb = textures.size(), bb = texels.size(), pb = perlin.size(), lb = lights.size(),
-                   gb = segments.size() / 2;
+                   gb = segments.size() / 2, mob = motions.size();
     if (sb + f.n_slots >= TPT_NO_RIGHT || mb + f.n_materials > TPT_PAYLOAD_MASK || bb + f.n_texels > 0xFFFFFFFFull ||
         eb + f.n_entries > 0xFFFFFFF0ull)
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: pool too large for 32-bit indices");
@@ -354,6 +378,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
         s.a += static_cast<uint32_t>(eb);
       } else {
         s.a = rebase_word(s.a, static_cast<uint32_t>(mb));
+        if (kind == TPT_SLOT_MSPHERE) s.f[0] = rebase_bits(s.f[0], static_cast<uint32_t>(mob));
       }
       slots.push_back(s);
     }
@@ -362,6 +387,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       tpt_entry en = f.entries[e];
       if (en.xform != 0xFFFFFFFFu) en.xform += static_cast<uint32_t>(xb);
       en.a = rebase_word(en.a, static_cast<uint32_t>(mb));
+      if (en.b == TPT_SLOT_MSPHERE) en.f[0] = rebase_bits(en.f[0], static_cast<uint32_t>(mob));
       entries.push_back(en);
     }
     xforms.insert(xforms.end(), f.xforms, f.xforms + f.n_xforms);
@@ -380,6 +406,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     texels.insert(texels.end(), f.texels, f.texels + f.n_texels);
     if (f.perlin) perlin.push_back(*f.perlin);
     lights.insert(lights.end(), f.lights, f.lights + f.n_lights);
+    if (f.n_motions) motions.insert(motions.end(), f.motions, f.motions + f.n_motions);
     scene_dev &d = scn[k];
     std::memset(&d, 0, sizeof(d));
     d.seg_begin = static_cast<uint32_t>(gb);
@@ -388,10 +415,12 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     d.n_lights = f.n_lights;
     d.light_mode = f.n_lights ? f.light_mode : 0;
     for (int q = 0; q < 6; ++q) d.bounds[q] = f.bounds[q];
+    motion[k] = f.n_motions > 0;
     // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
-    // close to each lane's own count only when the whole tree is small.
-    sweep[k] = f.n_slots <= 64;
-    simple[k] = f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT);
+    // close to each lane's own count only when the whole tree is small.  (The
+    // sweep kernel is built without the ray-time replay moving spheres need.)
+    sweep[k] = f.n_slots <= 64 && !motion[k];
+    simple[k] = !motion[k] && (f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT));
     for (uint32_t t = 0; t < f.n_textures; ++t) simple[k] = simple[k] && f.textures[t].kind == TPT_TEX_CONSTANT;
     nslots[k] = f.n_slots;
     nleaves[k] = f.n_leaves;
@@ -411,6 +440,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       (rc = upload(&s->texels, texels.data(), texels.size())) ||
       (rc = upload(&s->perlin, perlin.data(), perlin.size())) ||
       (rc = upload(&s->lights, lights.data(), lights.size())) ||
+      (rc = upload(&s->motions, motions.data(), motions.size())) ||
       (rc = upload(&s->d_scenes, scn.data(), scn.size()))) {
     free_scene(s);
     delete s;
@@ -423,6 +453,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   s->sweep = sweep;
   s->sph = sph;
   s->simple = simple;
+  s->motion = motion;
   s->n_slots_of = nslots;
   s->n_leaves_of = nleaves;
   if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
@@ -511,8 +542,8 @@ int tpt_scene_destroy(tpt_scene *s) {
   return TPT_OK;
 }
 
-int tpt_trace_closest_record(tpt_scene *s, const float *d_rays, uint64_t n, float t_min, float t_max,
-                             int32_t *d_leaf, float *d_t, float *d_rec, int32_t *d_mat, void *stream) {
+int tpt_trace_closest_timed(tpt_scene *s, const float *d_rays, const float *d_time, uint64_t n, float t_min,
+                            float t_max, int32_t *d_leaf, float *d_t, float *d_rec, int32_t *d_mat, void *stream) {
   if (!s || (n && (!d_rays || !d_leaf || !d_t))) return tpt::fail(TPT_ERR_INVALID, "tpt_trace_closest: null argument");
   if (n == 0) return TPT_OK;
   const uint64_t blocks = (n + 255) / 256;
@@ -520,14 +551,19 @@ int tpt_trace_closest_record(tpt_scene *s, const float *d_rays, uint64_t n, floa
   const scene_view S = s->view(0);
   if (s->sweep[0])
     hipLaunchKernelGGL(k_trace_closest<true>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, n, t_min, t_max, d_leaf, d_t,
-                       d_rec, d_mat);
+                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
+                       d_leaf, d_t, d_rec, d_mat);
   else
     hipLaunchKernelGGL(k_trace_closest<false>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, n, t_min, t_max, d_leaf, d_t,
-                       d_rec, d_mat);
+                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
+                       d_leaf, d_t, d_rec, d_mat);
   HIP_TRY(hipGetLastError());
   return TPT_OK;
+}
+
+int tpt_trace_closest_record(tpt_scene *s, const float *d_rays, uint64_t n, float t_min, float t_max,
+                             int32_t *d_leaf, float *d_t, float *d_rec, int32_t *d_mat, void *stream) {
+  return tpt_trace_closest_timed(s, d_rays, nullptr, n, t_min, t_max, d_leaf, d_t, d_rec, d_mat, stream);
 }
 
 int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min, float t_max, int32_t *d_leaf,
@@ -734,6 +770,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   for (uint32_t j = 0; j < n_jobs; ++j) {
     B.sph = B.sph && s->sph[jobs[j].scene];
     B.simple = B.simple && s->simple[jobs[j].scene];
+    if (s->motion[jobs[j].scene]) B.P.motion = 1;
   }
   B.multi = multi;
   B.guided = guide != nullptr;

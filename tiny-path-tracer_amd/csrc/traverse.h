@@ -130,6 +130,22 @@ __device__ __forceinline__ bool sphere_t(float cx, float cy, float cz, float r, 
   return false;
 }
 
+// moving_sphere::center (headers/sphere.h:28-31), in float.
+__device__ __forceinline__ v3 motion_center(const tpt_motion &m, float time) {
+  const float s = (time - m.time0) / (m.time1 - m.time0);
+  return v3{m.center0[0] + s * (m.center1[0] - m.center0[0]), m.center0[1] + s * (m.center1[1] - m.center0[1]),
+            m.center0[2] + s * (m.center1[2] - m.center0[2])};
+}
+
+// moving_sphere::hit (src/sphere.cc:45-75): sphere::hit's arithmetic about
+// the centre at the ray's time.
+__device__ __forceinline__ bool msphere_t(const scene_view &S, float idx_bits, v3 o, v3 d, float time, float tmin,
+                                          float tmax, float *t) {
+  const tpt_motion m = S.motions[__float_as_uint(idx_bits)];
+  const v3 c = motion_center(m, time);
+  return sphere_t(c.x, c.y, c.z, m.radius, o, d, tmin, tmax, t);
+}
+
 // f = (k, a0, a1, b0, b1); axis = normal axis; in-plane axes (i0, i1) are
 // (1,2) for yz, (0,2) for xz, (0,1) for xy.
 __device__ __forceinline__ bool rect_t(const float *f, uint32_t axis, v3 o, v3 d, float tmin, float tmax,
@@ -177,8 +193,8 @@ __device__ __forceinline__ void to_world(const tpt_xform &x, v3 *p, v3 *n) {
 }
 
 // LIST leaf: primitives in order, t_max shrinking to each accepted t.
-__device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 o, v3 d, float tmin, float tmax,
-                                       float *t, uint32_t *entry, uint32_t *mat) {
+__device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 o, v3 d, float time, float tmin,
+                                       float tmax, float *t, uint32_t *entry, uint32_t *mat) {
   const uint32_t first = s.a;
   const uint32_t count = __float_as_uint(s.f[1]);
   float cur = tmax;
@@ -194,8 +210,9 @@ __device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 
       cached = en.xform;
     }
     float th;
-    const bool h = en.b == TPT_SLOT_SPHERE ? sphere_t(en.f[0], en.f[1], en.f[2], en.f[3], lo, ld, tmin, cur, &th)
-                                           : rect_t(en.f, (en.a >> 28) & 3u, lo, ld, tmin, cur, &th);
+    const bool h = en.b == TPT_SLOT_SPHERE    ? sphere_t(en.f[0], en.f[1], en.f[2], en.f[3], lo, ld, tmin, cur, &th)
+                   : en.b == TPT_SLOT_MSPHERE ? msphere_t(S, en.f[0], lo, ld, time, tmin, cur, &th)
+                                              : rect_t(en.f, (en.a >> 28) & 3u, lo, ld, tmin, cur, &th);
     if (h) {
       cur = th;
       found = true;
@@ -209,8 +226,8 @@ __device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 
 
 // SPH: the scene's leaves are all spheres (no rect or list code is compiled).
 template <bool SPH = false>
-__device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d, float tmin,
-                                       float tmax, float *t, uint32_t *entry, uint32_t *mat) {
+__device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d, float time,
+                                       float tmin, float tmax, float *t, uint32_t *entry, uint32_t *mat) {
   if (SPH || kind == TPT_SLOT_SPHERE) {
     *entry = kNone;
     *mat = s.a;
@@ -221,14 +238,19 @@ __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uin
     *mat = s.a;
     return rect_t(s.f, (s.a >> 28) & 3u, o, d, tmin, tmax, t);
   }
-  return list_t(S, s, o, d, tmin, tmax, t, entry, mat);
+  if (kind == TPT_SLOT_MSPHERE) {
+    *entry = kNone;
+    *mat = s.a;
+    return msphere_t(S, s.f[0], o, d, time, tmin, tmax, t);
+  }
+  return list_t(S, s, o, d, time, tmin, tmax, t, entry, mat);
 }
 
 // Exact replay of the bvh_node recursion for one segment: evaluates each node
 // with a fresh result, combining (left, right) as `left.t < right.t ? left :
 // right`.  Stack entries: a pending right child, or a finished left result.
 template <bool SPH = false>
-__device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
+__device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv, float time,
                                                         float tmin, float tmax, hit_t *out) {
   struct frame {
     uint32_t tag;  // bit31 set: finished-left result; else pending right slot
@@ -256,7 +278,7 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
       } else {
         float t;
         uint32_t entry, mat;
-        have = leaf_t<SPH>(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat);
+        have = leaf_t<SPH>(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat);
         acc = hit_t{t, i, entry, mat};
       }
       break;
@@ -291,7 +313,8 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
 // when a leaf returned a NaN t (the caller then uses segment_exact).
 template <bool SPH = false>
 __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
-                                             float tmin, float tmax, hit_t *out, bool *saw_nan, uint32_t *nodes) {
+                                             float time, float tmin, float tmax, hit_t *out, bool *saw_nan,
+                                             uint32_t *nodes) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t i = start;
@@ -306,7 +329,7 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
     } else {
       float t;
       uint32_t entry, mat;
-      if (leaf_t<SPH>(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
+      if (leaf_t<SPH>(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat)) {
         if (t != t) *saw_nan = true;
         if (!have || t <= best.t) {
           best = hit_t{t, i, entry, mat};
@@ -340,7 +363,7 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // slot kind does not diverge.  Lanes with `live` false take no part.
 // Must be called by all lanes of the wave together.
 __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t start, uint32_t end, bool live, v3 o, v3 d,
-                                              v3 inv, float tmin, float tmax, hit_t *out, bool *saw_nan,
+                                              v3 inv, float time, float tmin, float tmax, hit_t *out, bool *saw_nan,
                                               uint32_t *nodes) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
@@ -358,7 +381,7 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
       } else {
         float t;
         uint32_t entry, mat;
-        if (leaf_t(S, s, kind, o, d, tmin, tmax, &t, &entry, &mat)) {
+        if (leaf_t(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat)) {
           if (t != t) *saw_nan = true;
           if (!have || t <= best.t) {
             best = hit_t{t, i, entry, mat};
@@ -384,8 +407,8 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
 // NaN t) sends the segment to segment_exact, which keeps the reference's slab
 // updates verbatim.
 template <bool SWEEP = false, bool SPH = false>
-__device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float tmin, float tmax, hit_t *out,
-                                            uint32_t *nodes, bool live = true) {
+__device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float time, float tmin, float tmax,
+                                            hit_t *out, uint32_t *nodes, bool live = true) {
   const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
   float cur = tmax;
   bool any = false;
@@ -397,12 +420,12 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, flo
     const bool bounded = tmin == tmin && cur == cur;
     bool got = false;
     if (SWEEP) {
-      got = segment_sweep(S, start, end, live && bounded, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+      got = segment_sweep(S, start, end, live && bounded, o, d, inv, time, tmin, cur, &h, &saw_nan, nodes);
     } else {
       if (!live) continue;
-      if (bounded) got = segment_fast<SPH>(S, start, end, o, d, inv, tmin, cur, &h, &saw_nan, nodes);
+      if (bounded) got = segment_fast<SPH>(S, start, end, o, d, inv, time, tmin, cur, &h, &saw_nan, nodes);
     }
-    if (live && (saw_nan || !bounded)) got = segment_exact<SPH>(S, start, o, d, inv, tmin, cur, &h);
+    if (live && (saw_nan || !bounded)) got = segment_exact<SPH>(S, start, o, d, inv, time, tmin, cur, &h);
     if (got) {
       *out = h;
       cur = h.t;
@@ -420,8 +443,8 @@ __device__ __forceinline__ uint32_t leaf_id(const scene_view &S, const hit_t &h)
 }
 
 // Full hit_record of the winner: point, normal (flip applied), u, v.
-__device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, v3 o, v3 d, v3 *p, v3 *n, float *u,
-                                           float *v) {
+__device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, v3 o, v3 d, float time, v3 *p, v3 *n,
+                                           float *u, float *v) {
   const slot_t s = load_slot(&S.slots[h.slot]);
   uint32_t kind = s.b >> 28;
   float f[5] = {s.f[0], s.f[1], s.f[2], s.f[3], s.f[4]};
@@ -439,7 +462,17 @@ __device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, 
   const float t = h.t;
   v3 pt{lo.x + t * ld.x, lo.y + t * ld.y, lo.z + t * ld.z};
   v3 nn;
-  if (kind == TPT_SLOT_SPHERE) {
+  if (kind == TPT_SLOT_MSPHERE) {
+    // moving_sphere::hit (src/sphere.cc:53-57): normal about the centre at
+    // the ray's time, uv about center0.
+    const tpt_motion m = S.motions[__float_as_uint(f[0])];
+    const v3 c = motion_center(m, time);
+    const float r = m.radius;
+    const v3 q{(pt.x - m.center0[0]) / r, (pt.y - m.center0[1]) / r, (pt.z - m.center0[2]) / r};
+    *u = static_cast<float>(static_cast<double>(atan2f(q.z, q.x)) / (2 * M_PI)) + 0.5f;
+    *v = static_cast<float>(static_cast<double>(asinf(q.y)) / M_PI) + 0.5f;
+    nn = v3{(pt.x - c.x) / r, (pt.y - c.y) / r, (pt.z - c.z) / r};
+  } else if (kind == TPT_SLOT_SPHERE) {
     const float r = f[3];
     const v3 q{(pt.x - f[0]) / r, (pt.y - f[1]) / r, (pt.z - f[2]) / r};
     // get_uv_map (headers/utils.h:38-43); atan2f/asinf are the device's.

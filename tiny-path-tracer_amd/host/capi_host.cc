@@ -74,6 +74,8 @@ int tpt_host_scene_info(const tpt_host_scene *hs, tpt_scene_info *info) {
   info->vfov = hs->spec.vfov;
   info->aperture = hs->spec.aperture;
   info->focus = hs->spec.focus;
+  info->time0 = hs->spec.t0;
+  info->time1 = hs->spec.t1;
   return TPT_OK;
 }
 

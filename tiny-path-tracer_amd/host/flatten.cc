@@ -169,12 +169,11 @@ class flattener {
     } else if (auto *l = dynamic_cast<const hitable_list *>(h)) {
       for (int i = 0; i < l->list_size_; ++i) collect(l->list_[i], chain, flip, refs);
     } else if (dynamic_cast<const sphere *>(h) || dynamic_cast<const xy_rect *>(h) ||
-               dynamic_cast<const xz_rect *>(h) || dynamic_cast<const yz_rect *>(h)) {
+               dynamic_cast<const xz_rect *>(h) || dynamic_cast<const yz_rect *>(h) ||
+               dynamic_cast<const moving_sphere *>(h)) {
       if (chain.size() > TPT_XFORM_MAX_OPS)
         throw flatten_error(TPT_ERR_UNSUPPORTED, "more than 4 nested translate/rotate_y wrappers");
       refs->push_back(prim_ref{h, chain, flip});
-    } else if (dynamic_cast<const moving_sphere *>(h)) {
-      throw flatten_error(TPT_ERR_UNSUPPORTED, "moving_sphere is not on the device path");
     } else if (dynamic_cast<const constant_medium *>(h)) {
       throw flatten_error(TPT_ERR_UNSUPPORTED, "constant_medium is not on the device path");
     } else {
@@ -193,6 +192,17 @@ class flattener {
       f[4] = 0.0f;
       mat = material_id(s->mat_ptr_);
       *kind = TPT_SLOT_SPHERE;
+    } else if (auto *ms = dynamic_cast<const moving_sphere *>(h)) {
+      tpt_motion m{};
+      for (int i = 0; i < 3; ++i) m.center0[i] = ms->center0_.e[i], m.center1[i] = ms->center1_.e[i];
+      m.radius = ms->radius_;
+      m.time0 = ms->time0_;
+      m.time1 = ms->time1_;
+      f[0] = bitsf(static_cast<uint32_t>(out_->motions.size()));
+      f[1] = f[2] = f[3] = f[4] = 0.0f;
+      out_->motions.push_back(m);
+      mat = material_id(ms->mat_ptr_);
+      *kind = TPT_SLOT_MSPHERE;
     } else {
       *kind = TPT_SLOT_RECT;
       if (auto *r = dynamic_cast<const xy_rect *>(h)) {
@@ -385,6 +395,8 @@ tpt_flat_scene flat_scene::view() const {
   v.n_leaves = n_leaves;
   v.max_depth = max_depth;
   for (int i = 0; i < 6; ++i) v.bounds[i] = bounds[i];
+  v.motions = motions.data();
+  v.n_motions = static_cast<uint32_t>(motions.size());
   return v;
 }
 

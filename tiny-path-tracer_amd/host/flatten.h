@@ -33,6 +33,7 @@ struct flat_scene {
   std::vector<uint8_t> texels;
   std::vector<tpt_perlin> perlin;  // 0 or 1 element
   std::vector<tpt_light> lights;
+  std::vector<tpt_motion> motions;
   uint32_t light_mode = 0;
   uint32_t n_leaves = 0;
   uint32_t max_depth = 0;

@@ -23,7 +23,8 @@ namespace tpt {
 
 const char *scene_names() {
   return "random_spheres,random_spheres_textured,cornell555,random_spheres_n<N>,ref_cornell_box,"
-         "ref_sphere_cornell_box,ref_two_perlin_spheres,ref_light_spheres,ref_two_checker_spheres,box_list";
+         "ref_sphere_cornell_box,ref_two_perlin_spheres,ref_light_spheres,ref_two_checker_spheres,box_list,"
+         "ref_random_scene";
 }
 
 std::vector<unsigned char> noise_image(int width, int height, uint64_t seed) {
@@ -43,13 +44,16 @@ std::vector<unsigned char> noise_image(int width, int height, uint64_t seed) {
 
 namespace {
 
-void make_camera(scene_spec *s, vec3 from, vec3 at, float vfov, float aperture, float focus) {
+void make_camera(scene_spec *s, vec3 from, vec3 at, float vfov, float aperture, float focus, float t0 = 0.0f,
+                 float t1 = 0.0f) {
   const float aspect = static_cast<float>(s->width) / static_cast<float>(s->height);
   for (int k = 0; k < 3; ++k) s->lookfrom[k] = from[k], s->lookat[k] = at[k];
   s->vfov = vfov;
   s->aperture = aperture;
   s->focus = focus;
-  s->cam.reset(new camera_with_blur(from, at, vec3(0, 1, 0), vfov, aspect, aperture, focus, 0.0f, 0.0f));
+  s->t0 = t0;
+  s->t1 = t1;
+  s->cam.reset(new camera_with_blur(from, at, vec3(0, 1, 0), vfov, aspect, aperture, focus, t0, t1));
 }
 
 bool build_now(const std::string &name, uint32_t seed, scene_spec *s, std::string *err) {
@@ -110,6 +114,18 @@ bool build_now(const std::string &name, uint32_t seed, scene_spec *s, std::strin
     s->max_depth = 15;
     s->background = 1.0f;
     make_camera(s, vec3(13, 2, 3), vec3(0, 0, 0), 20.0f, 0.0f, 10.0f);
+    return true;
+  }
+  if (name == "ref_random_scene") {
+    // the reference's random_scene with the camera its driver leaves commented
+    // (main.cpp:82-84) and the shutter of config.ini's [CAM_MOTION] (0, 1)
+    s->world = tpt_scene_ref_random_scene();
+    s->width = s->height = 400;
+    s->spp = 16;
+    s->max_depth = 50;
+    s->background = 1.0f;
+    const vec3 from(13, 2, 3), at(0, 0, 0);
+    make_camera(s, from, at, 20.0f, 0.0f, (from - at).length(), 0.0f, 1.0f);
     return true;
   }
   if (name == "ref_light_spheres") {

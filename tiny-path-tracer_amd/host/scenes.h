@@ -23,6 +23,7 @@ struct scene_spec {
   float background = 0.0f;
   // the camera's construction parameters (a driver rebuilds it with its own fov / aspect)
   float lookfrom[3] = {0, 0, 0}, lookat[3] = {0, 0, -1}, vfov = 90.0f, aperture = 0.0f, focus = 1.0f;
+  float t0 = 0.0f, t1 = 0.0f;  // shutter
   std::vector<unsigned char> image;  // texels owned for image_texture
 };
 
