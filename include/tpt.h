@@ -61,6 +61,7 @@ enum tpt_status {
 #define TPT_SLOT_RECT 2u
 #define TPT_SLOT_LIST 3u
 #define TPT_SLOT_MSPHERE 4u
+#define TPT_SLOT_MEDIUM 5u
 #define TPT_PAYLOAD_MASK 0x0FFFFFFFu
 #define TPT_NO_RIGHT 0x0FFFFFFFu
 
@@ -76,6 +77,11 @@ enum tpt_status {
  *          flip<<30; b payload = leaf id.
  *  MSPHERE: a moving_sphere (headers/sphere.h:23-40): f[0] = index into
  *          motions (as uint32 bits); a = material; b payload = leaf id.
+ *  MEDIUM: a constant_medium (headers/hitable.h:55-66): f[0] = first entry of
+ *          its boundary's primitives, f[1] = their count (both as uint32 bits),
+ *          f[2] = density; a = material (its isotropic phase function);
+ *          b payload = leaf id.  The boundary entries are evaluated like a LIST
+ *          (hitable_list order, shrinking t_max) and hold no MEDIUM.
  *  LIST:   a = first entry; f[0] = leaf id of the first entry (as uint32 bits),
  *          f[1] = entry count (as uint32 bits).  A leaf made of wrappers
  *          (translate / rotate_y / flip_normal), box and hitable_list around
@@ -91,10 +97,11 @@ typedef struct tpt_slot {
 
 /* 32-byte entry of a LIST leaf: one primitive seen through a wrapper chain.
  *  f[0..4] primitive parameters as in tpt_slot (sphere: centre, radius;
- *  rect: k, a0, a1, b0, b1; moving sphere: f[0] = motion index bits);
- *  xform = index into xforms or 0xFFFFFFFF;
+ *  rect: k, a0, a1, b0, b1; moving sphere: f[0] = motion index bits;
+ *  medium: f[0..2] as the MEDIUM slot);
+ *  xform = index into xforms or 0xFFFFFFFF (always none for a medium);
  *  a = material | axis<<28 | flip<<30 (flip = parity of flip_normal wrappers);
- *  b = TPT_SLOT_SPHERE, TPT_SLOT_RECT or TPT_SLOT_MSPHERE. */
+ *  b = TPT_SLOT_SPHERE, TPT_SLOT_RECT, TPT_SLOT_MSPHERE or TPT_SLOT_MEDIUM. */
 typedef struct tpt_entry {
   float f[5];
   uint32_t xform;
@@ -119,8 +126,10 @@ typedef struct tpt_xform {
 #define TPT_MAT_METAL 1u
 #define TPT_MAT_DIELECTRIC 2u
 #define TPT_MAT_DIFFUSE_LIGHT 3u
+#define TPT_MAT_ISOTROPIC 4u
 /* lambertian: tex = albedo texture.  metal: p[0..2] albedo, p[3] fuzz.
- * dielectric: p[0] ior.  diffuse_light: tex = emission texture. */
+ * dielectric: p[0] ior.  diffuse_light: tex = emission texture.
+ * isotropic (constant_medium's phase function): tex = albedo texture. */
 typedef struct tpt_material {
   uint32_t kind;
   uint32_t tex;
@@ -278,7 +287,12 @@ int tpt_trace_closest_record(tpt_scene *scene, const float *d_rays, uint64_t n, 
 
 /* Rays with a time (ray::time(), headers/ray.h): d_time = n floats, the time
  * each ray's moving spheres are placed at (NULL: time 0, the default of the
- * reference's ray constructor).  d_rec and d_mat may be NULL. */
+ * reference's ray constructor).  d_rec and d_mat may be NULL.
+ *
+ * constant_medium::hit draws random numbers (src/hitable.cc:98-128): in every
+ * closest-hit entry point, ray i's medium with leaf id L draws word k of
+ * Philox4x32-10(counter = (i mod 2^32, i / 2^32, 0, 2^20 + L), key = 0): word 0
+ * for the free path, words 1-3 for the arbitrary normal of a hit (u = v = 0). */
 int tpt_trace_closest_timed(tpt_scene *scene, const float *d_rays, const float *d_time, uint64_t n,
                             float t_min, float t_max, int32_t *d_leaf, float *d_t, float *d_rec,
                             int32_t *d_mat, void *stream);
@@ -308,6 +322,12 @@ typedef struct tpt_render_desc {
 /* flags: time every trace / shade launch with HIP events (fills the *_ms and
  * *_launches fields of tpt_render_stats). */
 #define TPT_RENDER_PROFILE 1u
+/* flags: constant_medium scatters as the book intends (isotropic: a direction
+ * from random_in_unit_sphere, throughput times the albedo).  Without it a
+ * medium behaves as the reference's does: isotropic::scatter does not
+ * override material::scatter (headers/material.h:74-80), so a path that the
+ * medium stops ends there, black. */
+#define TPT_RENDER_MEDIA_SCATTER 2u
 
 typedef struct tpt_render_stats {
   uint64_t paths;          /* camera paths completed */

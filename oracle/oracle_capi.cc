@@ -26,6 +26,11 @@ struct philox_rng : oracle::rng {
   oracle::path_stream s;
   double next() override { return s.next(); }
   void bounce(uint32_t b) override { s.set_bounce(b); }
+  // counter (pixel, sample, bounce, 2^20 + leaf), as the device draws them
+  double medium(uint32_t leaf, int k) override {
+    const oracle::philox_out o = oracle::philox4x32_10(s.pixel, s.sample, s.bounce, (1u << 20) + leaf, s.k0, s.k1);
+    return static_cast<double>(oracle::u01(o.v[k]));
+  }
 };
 
 struct mt_rng : oracle::rng {
@@ -157,9 +162,15 @@ int oracle_trace(const oracle_scene *s, const float *rays, uint64_t n, float tmi
   parallel_for(n, nthreads, [&](uint64_t i) {
     const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]), vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]),
                 time ? time[i] : 0.0f);
+    // media draws of ray i: counter (i mod 2^32, i / 2^32, 0, 2^20 + leaf), key 0
+    philox_rng M;
+    M.s.pixel = static_cast<uint32_t>(i);
+    M.s.sample = static_cast<uint32_t>(i >> 32);
+    M.s.k0 = M.s.k1 = 0;
+    M.s.set_bounce(0);
     hit_record hr;
     uint32_t lf = 0;
-    if (oracle::hit(s->spec.world, s->g, r, tmin, tmax, hr, 0, &lf)) {
+    if (oracle::hit(s->spec.world, s->g, r, tmin, tmax, hr, 0, &lf, &M)) {
       leaf[i] = static_cast<int32_t>(lf);
       t[i] = hr.t;
       if (rec) {
@@ -213,7 +224,7 @@ int oracle_make_rays(const oracle_scene *s, int kind, uint64_t n, uint64_t seed,
       for (int tries = 0; tries < 8; ++tries) {
         hit_record rec;
         uint32_t lf;
-        if (oracle::hit(s->spec.world, s->g, out, 0.001f, std::numeric_limits<float>::max(), rec, 0, &lf)) {
+        if (oracle::hit(s->spec.world, s->g, out, 0.001f, std::numeric_limits<float>::max(), rec, 0, &lf, &R)) {
           ray next;
           if (oracle::scatter_once(out, rec, s->g, s->spec.lights, R, &next)) {
             out = next;
@@ -300,10 +311,11 @@ int oracle_make_rays(const oracle_scene *s, int kind, uint64_t n, uint64_t seed,
 // thread-per-row generator state, main.cpp:157-159 + src/utils.cc:28-32).
 int oracle_render(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint64_t seed,
                   uint32_t sample_offset, float background, int rng_mode, int nthreads, float *sum,
-                  float *sumsq) {
+                  float *sumsq, int media_scatter) {
   oracle::render_opts o;
   o.max_depth = static_cast<int>(max_depth);
   o.background = background;
+  o.media_scatter = media_scatter != 0;
   parallel_for(h, nthreads, [&](uint64_t j) {
     mt_rng mt;
     philox_rng ph;

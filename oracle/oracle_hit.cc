@@ -30,6 +30,9 @@ uint32_t count_leaves(const hitable *h, graph_info *g) {
     n = count_leaves(t->ptr_, g);
   } else if (auto *r = dynamic_cast<const rotate_y *>(h)) {
     n = count_leaves(r->ptr_, g);
+  } else if (auto *cm = dynamic_cast<const constant_medium *>(h)) {
+    (void)count_leaves(cm->boundary_, g);  // indexed for the boundary's own hit(); the medium is one leaf
+    n = 1;
   } else {
     n = 1;
   }
@@ -55,6 +58,7 @@ void number_materials(const hitable *h, graph_info *g) {
     const material *m = nullptr;
     if (auto *s = dynamic_cast<const sphere *>(h)) m = s->mat_ptr_;
     if (auto *ms = dynamic_cast<const moving_sphere *>(h)) m = ms->mat_ptr_;
+    if (auto *cm = dynamic_cast<const constant_medium *>(h)) m = cm->phase_funcion_;
     if (auto *r1 = dynamic_cast<const xy_rect *>(h)) m = r1->mat_ptr_;
     if (auto *r2 = dynamic_cast<const xz_rect *>(h)) m = r2->mat_ptr_;
     if (auto *r3 = dynamic_cast<const yz_rect *>(h)) m = r3->mat_ptr_;
@@ -167,16 +171,16 @@ bool slab_hit(const AABB &box, const ray &r, float tmin, float tmax) {
 }
 
 bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float t_max, hit_record &rec,
-         uint32_t base, uint32_t *leaf) {
+         uint32_t base, uint32_t *leaf, rng *R) {
   if (auto *b = dynamic_cast<const bvh_node *>(h)) {
     // src/hitable.cc:63-90: both children with the caller's t_max; the left
     // record wins only when strictly closer.
     if (!slab_hit(b->box_, r, t_min, t_max)) return false;
     hit_record lrec, rrec;
     uint32_t lleaf = 0, rleaf = 0;
-    const bool hl = hit(b->left_, g, r, t_min, t_max, lrec, base, &lleaf);
+    const bool hl = hit(b->left_, g, r, t_min, t_max, lrec, base, &lleaf, R);
     const uint32_t rbase = b->right_ == b->left_ ? base : base + g.leaf_count.at(b->left_);
-    const bool hr = hit(b->right_, g, r, t_min, t_max, rrec, rbase, &rleaf);
+    const bool hr = hit(b->right_, g, r, t_min, t_max, rrec, rbase, &rleaf, R);
     if (hl && hr) {
       if (lrec.t < rrec.t) {
         rec = lrec;
@@ -208,7 +212,7 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
     uint32_t b0 = base;
     for (int i = 0; i < l->list_size_; ++i) {
       uint32_t lf = 0;
-      if (hit(l->list_[i], g, r, t_min, static_cast<float>(closest), tmp, b0, &lf)) {
+      if (hit(l->list_[i], g, r, t_min, static_cast<float>(closest), tmp, b0, &lf, R)) {
         any = true;
         closest = tmp.t;
         rec = tmp;
@@ -218,15 +222,15 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
     }
     return any;
   }
-  if (auto *bx = dynamic_cast<const box *>(h)) return hit(bx->list_ptr_, g, r, t_min, t_max, rec, base, leaf);
+  if (auto *bx = dynamic_cast<const box *>(h)) return hit(bx->list_ptr_, g, r, t_min, t_max, rec, base, leaf, R);
   if (auto *f = dynamic_cast<const flip_normal *>(h)) {
-    if (!hit(f->ptr_, g, r, t_min, t_max, rec, base, leaf)) return false;
+    if (!hit(f->ptr_, g, r, t_min, t_max, rec, base, leaf, R)) return false;
     rec.normal = -rec.normal;
     return true;
   }
   if (auto *t = dynamic_cast<const translate *>(h)) {
     const ray moved(r.origin() - t->offset_, r.direction(), r.time());
-    if (!hit(t->ptr_, g, moved, t_min, t_max, rec, base, leaf)) return false;
+    if (!hit(t->ptr_, g, moved, t_min, t_max, rec, base, leaf, R)) return false;
     rec.point += t->offset_;
     return true;
   }
@@ -238,7 +242,7 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
     o[2] = s * r.origin()[0] + c * r.origin()[2];
     d[0] = c * r.direction()[0] - s * r.direction()[2];
     d[2] = s * r.direction()[0] + c * r.direction()[2];
-    if (!hit(ry->ptr_, g, ray(o, d, r.time()), t_min, t_max, rec, base, leaf)) return false;
+    if (!hit(ry->ptr_, g, ray(o, d, r.time()), t_min, t_max, rec, base, leaf, R)) return false;
     vec3 p = rec.point, n = rec.normal;
     p[0] = c * rec.point[0] + s * rec.point[2];
     p[2] = -s * rec.point[0] + c * rec.point[2];
@@ -257,6 +261,35 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
     if (!moving_sphere_hit(*ms, r, t_min, t_max, rec)) return false;
     *leaf = base;
     return true;
+  }
+  if (auto *cm = dynamic_cast<const constant_medium *>(h)) {
+    // src/hitable.cc:98-128.  The boundary's hits: over the whole line, then
+    // from the first + 0.0001 (float + double, passed as float).
+    hit_record rec1, rec2;
+    uint32_t lf = 0;
+    if (!hit(cm->boundary_, g, r, std::numeric_limits<float>::lowest(), std::numeric_limits<float>::max(), rec1, 0,
+             &lf, nullptr))
+      return false;
+    if (!hit(cm->boundary_, g, r, static_cast<float>(rec1.t + 0.0001), std::numeric_limits<float>::max(), rec2, 0,
+             &lf, nullptr))
+      return false;
+    if (rec1.t < t_min) rec1.t = t_min;
+    if (rec2.t > t_max) rec2.t = t_max;
+    if (rec1.t >= rec2.t) return false;
+    const float distance_inside = (rec2.t - rec1.t) * r.direction().length();
+    const float hit_distance = static_cast<float>((-1 / cm->density_) * std::log(R->medium(base, 0)));
+    if (hit_distance < distance_inside) {
+      rec.t = rec1.t + hit_distance / r.direction().length();
+      rec.point = r.point_at_parameter(rec.t);
+      const double n0 = R->medium(base, 1), n1 = R->medium(base, 2), n2 = R->medium(base, 3);
+      rec.normal = vec3(static_cast<float>(n0), static_cast<float>(n1), static_cast<float>(n2));
+      rec.normal.normalize();
+      rec.u = rec.v = 0.0f;  // not set by the reference
+      rec.mat_ptr = cm->phase_funcion_;
+      *leaf = base;
+      return true;
+    }
+    return false;
   }
   bool ok = false;
   if (auto *q = dynamic_cast<const xy_rect *>(h)) {

@@ -262,7 +262,7 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
   R.bounce(static_cast<uint32_t>(depth + 1));
   hit_record rec;
   uint32_t leaf = 0;
-  if (!hit(world, g, r, 0.001f, kFltMax, rec, 0, &leaf)) {
+  if (!hit(world, g, r, 0.001f, kFltMax, rec, 0, &leaf, &R)) {
     return o.background > 0 ? sky(r, o.background) : vec3(0, 0, 0);
   }
   const material *m = rec.mat_ptr;
@@ -300,6 +300,14 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
     reflect_prob = refract(d, outward, ni_over_nt, refracted) ? schlick(cosine, ior) : 1.0f;
     const ray scattered(rec.point, R.next() < reflect_prob ? reflected : refracted, r.time());
     return vec3(1, 1, 1) * color(scattered, world, g, lights, depth + 1, o, R);
+  }
+  if (auto *iso = dynamic_cast<const isotropic *>(m)) {
+    // The reference's isotropic::scatter does not override material::scatter
+    // (headers/material.h:74-80): color() returns emitted (0).  The scatter
+    // mode follows src/material.cc:72-77 instead.
+    if (!o.media_scatter) return emitted;
+    const ray scattered(rec.point, in_unit_sphere(R), r.time());
+    return tex_value(iso->albedo_, rec.u, rec.v, rec.point) * color(scattered, world, g, lights, depth + 1, o, R);
   }
   auto *lam = dynamic_cast<const lambertian *>(m);
   if (!lam) return emitted;

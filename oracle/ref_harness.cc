@@ -86,6 +86,8 @@ struct tagger {
       tag(&s->mat_ptr_);
     } else if (auto *ms = dynamic_cast<moving_sphere *>(h)) {
       tag(&ms->mat_ptr_);
+    } else if (auto *cm = dynamic_cast<constant_medium *>(h)) {
+      tag(&cm->phase_funcion_);  // the medium is one leaf; its boundary is not walked
     } else if (auto *q = dynamic_cast<xy_rect *>(h)) {
       tag(&q->mat_ptr_);
     } else if (auto *q2 = dynamic_cast<xz_rect *>(h)) {
@@ -139,6 +141,8 @@ bool build(const std::string &name, ref_scene_t *s, hitable **world) {
     *world = tpt_scene_box_list();
   } else if (name == "ref_random_scene") {
     *world = tpt_scene_ref_random_scene();
+  } else if (name == "cornell_smoke") {
+    *world = tpt_scene_cornell_smoke(&lights);
   } else if (name == "native:cornell_box") {
     // The reference's own factory and its driver's light list (main.cpp:99-106).
     *world = cornell_box();

@@ -43,7 +43,7 @@ class Oracle:
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.oracle_make_rays.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64, _f32p, C.c_int]
         L.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
-                                    C.c_uint32, C.c_float, C.c_int, C.c_int, _f32p, C.c_void_p]
+                                    C.c_uint32, C.c_float, C.c_int, C.c_int, _f32p, C.c_void_p, C.c_int]
         L.oracle_philox.argtypes = [C.c_uint32] * 6 + [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")]
         L.oracle_slab.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_float, C.c_float]
         L.oracle_prim_hit.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_float, C.c_float, _f32p, _f32p]
@@ -130,12 +130,12 @@ class OracleScene:
         return (leaf, t, rec, mat) if record else (leaf, t)
 
     def render(self, width, height, spp, max_depth=None, seed=1, sample_offset=0, background=None,
-               rng_mode=0, nthreads=None):
+               rng_mode=0, nthreads=None, media_scatter=False):
         s = np.zeros(width * height * 3, np.float32)
         sq = np.zeros(width * height * 3, np.float32)
         self.lib.L.oracle_render(self.h, width, height, spp, max_depth if max_depth is not None else self.max_depth,
                                  seed, sample_offset, self.background if background is None else background,
-                                 rng_mode, nthreads or _nthreads(), s, sq.ctypes.data)
+                                 rng_mode, nthreads or _nthreads(), s, sq.ctypes.data, 1 if media_scatter else 0)
         return s.reshape(height, width, 3), sq.reshape(height, width, 3)
 
 

@@ -116,6 +116,28 @@ def test_moving_spheres_flatten(tpt, oracle):
     assert info.time0 == 0.0 and info.time1 == 1.0
 
 
+def test_media_flatten(tpt, oracle):
+    # cornell_smoke: one LIST leaf (6 walls + 2 media) and, after it, each
+    # medium's boundary (a rotated, translated box: 6 rects) as entries
+    import ctypes as C
+
+    hs = tpt.HostScene("cornell_smoke")
+    f = hs.flat
+    assert f.n_leaves == oracle.scene("cornell_smoke").n_leaves == 8
+    assert f.n_slots == 1 and f.n_entries == 8 + 12
+    ent = (C.c_uint32 * (8 * f.n_entries)).from_address(f.entries)
+    words = np.frombuffer(bytes(ent), np.uint32).reshape(-1, 8)
+    kinds = words[:, 7]
+    assert list(kinds[:8]) == [2] * 6 + [5, 5]
+    first = words[6:8, 0]
+    assert list(first) == [8, 14] and list(words[6:8, 1]) == [6, 6]
+    assert np.all(np.frombuffer(words[6:8, 2].tobytes(), np.float32) == np.float32(0.01))
+    assert np.all(kinds[8:] == 2) and np.all(words[8:, 5] != 0xFFFFFFFF)  # boundary rects carry the box transform
+    mats = (C.c_uint32 * (8 * f.n_materials)).from_address(f.materials)
+    mk = np.frombuffer(bytes(mats), np.uint32).reshape(-1, 8)[:, 0]
+    assert list(mk) == [0, 0, 3, 0, 4, 4]  # green, red, light, white, two isotropic phase functions
+
+
 def test_unsupported_types_fail_cleanly(tpt):
     # the product scene library has no unsupported scene; the flattener's
     # refusal is exercised through a scene name that does not exist

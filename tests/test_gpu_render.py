@@ -17,9 +17,10 @@ def cam_for(tpt, name):
     return tpt.HostScene(name).camera
 
 
-def gpu_render(tpt, name, w, h, spp, seed, max_depth=None):
+def gpu_render(tpt, name, w, h, spp, seed, max_depth=None, media_scatter=False):
     sc = tpt.Scene.named(name)
-    s, sq, st = sc.render(width=w, height=h, spp=spp, seed=seed, max_depth=max_depth, camera=sc.host.camera)
+    s, sq, st = sc.render(width=w, height=h, spp=spp, seed=seed, max_depth=max_depth, camera=sc.host.camera,
+                          media_scatter=media_scatter)
     return s.cpu().numpy(), sq.cpu().numpy(), st
 
 
@@ -32,7 +33,7 @@ def zscores(s1, q1, n1, s2, q2, n2):
 
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 48, 32), ("random_spheres", 48, 16), ("ref_cornell_box", 40, 16),
                                         ("random_spheres_textured", 40, 8), ("box_list", 40, 8),
-                                        ("ref_random_scene", 48, 16)])
+                                        ("ref_random_scene", 48, 16), ("cornell_smoke", 40, 16)])
 def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
     s_g, q_g, st = gpu_render(tpt, name, w, w, spp, seed=11)
     s_c, q_c = oracle.scene(name).render(w, w, spp, seed=11)
@@ -46,7 +47,7 @@ def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
 
 
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 40, 64), ("random_spheres", 40, 32),
-                                        ("ref_random_scene", 40, 32)])
+                                        ("ref_random_scene", 40, 32), ("cornell_smoke", 40, 64)])
 def test_independent_streams_within_3_sigma(tpt, oracle, device, name, w, spp):
     s_g, q_g, _ = gpu_render(tpt, name, w, w, spp, seed=123)
     s_c, q_c = oracle.scene(name).render(w, w, spp, seed=456)
@@ -101,3 +102,18 @@ def test_queue_capacity_does_not_change_result(tpt, device):
     a, _, _ = sc.render(width=40, height=40, spp=4, seed=3, camera=cam)
     b, _, _ = sc.render(width=40, height=40, spp=4, seed=3, camera=cam, queue_capacity=333)
     assert bool((a == b).all())
+
+
+def test_media_scatter_mode_matches_oracle(tpt, oracle, device):
+    """TPT_RENDER_MEDIA_SCATTER (isotropic media scatter as the book intends)
+    against the oracle's scatter mode on the same streams, and brighter than
+    the reference's black media."""
+    w, spp = 40, 32
+    s_g, q_g, _ = gpu_render(tpt, "cornell_smoke", w, w, spp, seed=21, media_scatter=True)
+    s_c, q_c = oracle.scene("cornell_smoke").render(w, w, spp, seed=21, media_scatter=True)
+    close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
+    assert close.all(axis=2).mean() > 0.9
+    z = zscores(s_g, q_g, spp, s_c, q_c, spp)
+    assert np.mean(np.abs(z) > 4) < 0.01
+    s_b, _, _ = gpu_render(tpt, "cornell_smoke", w, w, spp, seed=21)
+    assert s_g.mean() > 1.02 * s_b.mean()

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 FULL = [("cornell555", 1 << 24), ("random_spheres", 1 << 24)]
 SMALL = ["ref_cornell_box", "box_list", "random_spheres_textured", "ref_light_spheres", "ref_two_perlin_spheres",
-         "ref_sphere_cornell_box", "random_spheres_n20000"]
+         "ref_sphere_cornell_box", "random_spheres_n20000", "cornell_smoke"]
 CHUNK = 1 << 22
 
 

@@ -123,3 +123,44 @@ def test_bvh_closest_equals_brute_force(oracle, tpt):
     ground = ids[np.argmax(spheres[:, 3])]
     hit = sel & (leaf >= 0) & (leaf != ground)
     assert np.allclose(t[hit], order[hit, 0], rtol=1e-4)
+
+
+def test_media_free_path_law(oracle):
+    """A constant_medium of density rho stops a ray that crosses it over a
+    parameter interval (t1, t2) with probability 1 - exp(-rho |d| (t2 - t1))
+    (the free path -ln(u)/rho, src/hitable.cc:114-116), at a distance that is
+    exponential in length.  Rays from the camera through the white smoke box of
+    cornell_smoke; (t1, t2) from the box's own transform in float64."""
+    s = oracle.scene("cornell_smoke")
+    rng = np.random.default_rng(17)
+    n = 40000
+    ang = np.deg2rad(np.float32(-18.0))
+    c, sn = np.cos(ang), np.sin(ang)
+    off = np.array([130.0, 0.0, 65.0])
+    # targets inside the box in its frame, mapped to world: p = R^-1 q + off
+    q = rng.uniform(20, 145, (n, 3))
+    p = np.stack([c * q[:, 0] + sn * q[:, 2], q[:, 1], -sn * q[:, 0] + c * q[:, 2]], 1) + off
+    o = np.array([278.0, 278.0, -800.0])
+    d = p - o
+    rays = np.concatenate([np.repeat(o[:, None], n, 1), d.T]).astype(np.float32)
+    # the box interval in the local frame: o' = R (o - off), d' = R d
+    def rot(v):
+        return np.stack([c * v[..., 0] - sn * v[..., 2], v[..., 1], sn * v[..., 0] + c * v[..., 2]], -1)
+    d64 = rays[3:].T.astype(np.float64)
+    lo_, ld_ = rot(o - off), rot(d64)
+    with np.errstate(divide="ignore"):
+        ta, tb = (0.0 - lo_) / ld_, (165.0 - lo_) / ld_
+    t1 = np.max(np.minimum(ta, tb), 1)
+    t2 = np.min(np.maximum(ta, tb), 1)
+    L = np.linalg.norm(d64, axis=1)
+    pstop = 1 - np.exp(-0.01 * L * (t2 - t1))
+    leaf, t = s.trace(rays)
+    stopped = leaf == 6
+    z = (stopped.sum() - pstop.sum()) / np.sqrt((pstop * (1 - pstop)).sum())
+    assert abs(z) < 4, z
+    # free path lengths of the stopped rays: exponential(rate 0.01) truncated at the chord
+    x = (t[stopped] - t1[stopped]) * L[stopped]
+    chord = (t2 - t1)[stopped] * L[stopped]
+    mean_trunc = 100.0 - chord * np.exp(-0.01 * chord) / (1 - np.exp(-0.01 * chord))
+    se = np.sqrt(np.sum(100.0 ** 2 * np.ones_like(x))) / len(x)
+    assert abs(x.mean() - mean_trunc.mean()) < 5 * se, (x.mean(), mean_trunc.mean())

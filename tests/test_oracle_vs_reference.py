@@ -72,3 +72,26 @@ def test_moving_spheres_equal_reference(oracle, reference):
     l0 = s.trace(rays, time=np.zeros(rays.shape[1], np.float32))[0]
     l1 = s.trace(rays, time=np.ones(rays.shape[1], np.float32))[0]
     assert np.mean(l0 != l1) > 0.005
+
+
+def test_media_match_reference(oracle, reference):
+    """constant_medium::hit (src/hitable.cc:98-128) draws from the reference's
+    sequential generator, so single rays cannot be compared; instead: rays on
+    which neither side's media stop agree bit for bit (boundaries, clipping
+    and the rest of the list), and each medium stops rays equally often."""
+    s = oracle.scene("cornell_smoke")
+    h = reference.scene("cornell_smoke")
+    assert s.n_leaves == reference.leaf_count(h) == 8
+    media = [6, 7]  # the two constant_media follow the six walls in the list
+    for kind in range(3):
+        rays = s.make_rays(kind, 1 << 15, seed=311 + kind)
+        a_leaf, a_t = s.trace(rays)
+        b_leaf, b_t = reference.trace(h, rays)
+        calm = ~np.isin(a_leaf, media) & ~np.isin(b_leaf, media)
+        assert calm.mean() > 0.6
+        assert np.array_equal(a_leaf[calm], b_leaf[calm]), f"kind {kind}"
+        assert np.array_equal(a_t[calm].view(np.uint32), b_t[calm].view(np.uint32)), f"kind {kind}"
+        for m in media:
+            na, nb = (a_leaf == m).sum(), (b_leaf == m).sum()
+            # two binomial counts of the same (unknown) rate
+            assert abs(na - nb) <= 4 * np.sqrt(na + nb + 1), (kind, m, na, nb)

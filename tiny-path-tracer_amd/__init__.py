@@ -311,7 +311,7 @@ class Scene:
 
     def render(self, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0, background=None,
                guiding=0, queue_capacity=0, sum_buf=None, sumsq_buf=None, stream=None, camera=None,
-               profile=False, guide=None, guide_vertices=0):
+               profile=False, guide=None, guide_vertices=0, media_scatter=False):
         """Renders into (and returns) device tensors sum, sumsq of shape (H, W, 3);
         row 0 is the bottom image row as in the reference."""
         torch = _torch()
@@ -326,7 +326,7 @@ class Scene:
         desc.background = info.background if background is None else background
         desc.guiding = guiding
         desc.queue_capacity = queue_capacity
-        desc.flags = 1 if profile else 0
+        desc.flags = (1 if profile else 0) | (2 if media_scatter else 0)
         desc.guide_vertices = guide_vertices
         shape = (desc.height, desc.width, 3)
         if sum_buf is None:

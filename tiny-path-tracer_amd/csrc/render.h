@@ -109,6 +109,8 @@ struct render_params {
   float4 *rec_side;         // per recorded path: 2 vertices x 5 float4
   float4 *rec_rows;         // training rows: 2 per recorded path, 4 float4 each
   uint32_t motion;          // some scene has moving spheres: rays need their time
+  uint32_t media;           // some scene has constant_medium: queries need their media draws
+  uint32_t media_scatter;   // TPT_RENDER_MEDIA_SCATTER
 };
 
 constexpr uint32_t REC_VERTS = 2;
@@ -339,6 +341,19 @@ __device__ __forceinline__ float ray_time(const render_params &P, const item_ctx
                             static_cast<double>(xi) * static_cast<double>(J.cam.time1 - J.cam.time0));
 }
 
+// Closest-hit context of a ray traced at `depth` on item c: its time, and its
+// media draw from the stream of the bounce that shades the hit (counter
+// (pixel, sample, depth + 1, 2^20 + leaf), key = the job's seed).
+__device__ __forceinline__ ray_ctx make_ray_ctx(const render_params &P, const item_ctx &c, uint32_t depth) {
+  ray_ctx rc{0.0f, 0u, 0u, 0u, 0u, 0u};
+  if (P.motion) rc.time = ray_time(P, c);
+  if (P.media) {
+    const uint64_t seed = P.jobs[c.job].seed;
+    rc = ray_ctx{rc.time, c.px, c.smp, depth + 1, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+  }
+  return rc;
+}
+
 template <bool MULTI>
 __device__ __forceinline__ path raygen_item(const render_params &P, const pass_hdr &H, uint32_t item) {
   return raygen(P, item, item_in_entry(P, find_entry<MULTI>(P, H.n_ent, item), item));
@@ -368,15 +383,17 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
 
 // SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  SPH:
 // every leaf of the scene (or pool) is a sphere; without the rect and list
-// code the kernel fits 64 registers, 8 waves per SIMD.  The slot/entry arrays come
+// code the kernel fits 64 registers, 8 waves per SIMD.  MEDIA: the scenes hold
+// constant_media (their code costs registers, so only this variant has it).  The slot/entry arrays come
 // in as separate restrict arguments so that wave-uniform loads from them can
 // use the scalar data path.
-template <bool SWEEP, bool MULTI, bool SPH = false>
+template <bool SWEEP, bool MULTI, bool SPH = false, bool MEDIA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : 1))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
   static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");
   static_assert(!(SWEEP && SPH), "the sweep takes every leaf kind");
+  static_assert(!(MEDIA && (SWEEP || SPH)), "media go through the general walk");
   S.slots = slots;
   S.entries = entries;
   __shared__ uint32_t offs[NSUB];
@@ -413,11 +430,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
     const uint32_t ent = MULTI ? __float_as_uint(b.w) >> 16 : 0u;
     const uint32_t job = MULTI ? P.pass[ent].job : 0u;
     const scene_view Sj = view_of<MULTI>(S, P, job);
-    // (scenes with moving spheres use neither the sweep nor the sphere-only walk)
-    const float time =
-        !SWEEP && !SPH && P.motion && valid ? ray_time(P, item_in_entry(P, ent, __float_as_uint(a.w))) : 0.0f;
-    const bool got = trace_world<SWEEP, SPH>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, time, 0.001f, kFltMax, &h,
-                                             &st[ST_NODES], valid);
+    // (scenes with moving spheres or media use neither the sweep nor the
+    // sphere-only walk)
+    const ray_ctx rc = !SWEEP && !SPH && (P.motion | P.media) && valid
+                           ? make_ray_ctx(P, item_in_entry(P, ent, __float_as_uint(a.w)), depth)
+                           : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u};
+    const bool got = trace_world<SWEEP, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax,
+                                                    &h, &st[ST_NODES], valid);
     const float4 c = SWEEP || !valid ? c0 : in[2][src];
     if (valid) {
       ++st[ST_SEGMENTS];
@@ -565,7 +584,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
       } else {
         v3 pt, n;
         float u, v;
-        hit_record(S, h, p.o, p.d, !SIMPLE && P.motion ? ray_time(P, c) : 0.0f, &pt, &n, &u, &v);
+        hit_record(S, h, p.o, p.d,
+                   !SIMPLE && (P.motion | P.media) ? make_ray_ctx(P, c, p.depth) : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u},
+                   &pt, &n, &u, &v);
         const tpt_material M = S.materials[h.mat];
         stream R;
         R.init(c.px, c.smp, p.depth + 1, J.seed);
@@ -582,6 +603,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
           nd = refl + M.p[3] * in_unit_sphere(R);
           if (dot(nd, n) > 0.0f) {
             p.thr = p.thr * v3{M.p[0], M.p[1], M.p[2]};
+            cont = true;
+          } else {
+            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
+          }
+        } else if (M.kind == TPT_MAT_ISOTROPIC) {
+          // constant_medium's phase function.  The reference's
+          // isotropic::scatter does not override material::scatter
+          // (headers/material.h:74-80), so color() returns emitted (0): the
+          // path ends black.  TPT_RENDER_MEDIA_SCATTER scatters as the book
+          // intends (src/material.cc:72-77): random_in_unit_sphere, albedo.
+          if (!SIMPLE && P.media_scatter) {
+            nd = in_unit_sphere(R);
+            p.thr = p.thr * tex_value<SIMPLE>(S, M.tex, u, v, pt);
             cont = true;
           } else {
             p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
@@ -702,8 +736,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       v3 pt, n;
       float u, v;
-      const float time = P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f;
-      hit_record(S, h, p.o, p.d, time, &pt, &n, &u, &v);
+      // (the diffuse queue holds lambertian hits only: no media)
+      const ray_ctx rc{P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f, 0u, 0u, 0u, 0u, 0u};
+      hit_record(S, h, p.o, p.d, rc, &pt, &n, &u, &v);
       const v3 wo = -unit(p.d);
       const float pv[3] = {pt.x, pt.y, pt.z};
       const float *bounds = P.scenes[P.jobs[MULTI ? P.pass[p.ent].job : 0u].scene].bounds;
@@ -752,7 +787,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
       v3 pt, n;
       float u, v;
-      hit_record(S, h, p.o, p.d, P.motion ? ray_time(P, c) : 0.0f, &pt, &n, &u, &v);
+      hit_record(S, h, p.o, p.d, ray_ctx{P.motion ? ray_time(P, c) : 0.0f, 0u, 0u, 0u, 0u, 0u}, &pt, &n, &u, &v);
       vmf_mix mix;
       const bool finite = decode_mix_bf16(reinterpret_cast<const uint16_t *>(outs + threadIdx.x * 10), &mix);
       mix_norms(&mix);

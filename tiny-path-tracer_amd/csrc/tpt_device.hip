@@ -19,7 +19,7 @@ using namespace tptd;
 namespace {
 
 // -------------------------------------------------------- closest hit API ---
-template <bool SWEEP>
+template <bool SWEEP, bool MEDIA>
 __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_slot *__restrict__ slots,
                                                        const tpt_entry *__restrict__ entries,
                                                        const float *__restrict__ rays,
@@ -32,15 +32,16 @@ __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_s
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool live = i < n;
   v3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f};
-  float time = 0.0f;
+  // media draws: counter (i mod 2^32, i / 2^32, 0, 2^20 + leaf), key 0
+  ray_ctx rc{0.0f, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0u, 0u, 0u};
   if (live) {
     o = v3{rays[i], rays[n + i], rays[2 * n + i]};
     d = v3{rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]};
-    if (times) time = times[i];
+    if (times) rc.time = times[i];
   }
   hit_t h;
   uint32_t nodes = 0;
-  const bool got = trace_world<SWEEP>(S, o, d, time, tmin, tmax, &h, &nodes, live);
+  const bool got = trace_world<SWEEP, false, MEDIA>(S, o, d, rc, tmin, tmax, &h, &nodes, live);
   if (!live) return;
   if (got) {
     leaf[i] = static_cast<int32_t>(leaf_id(S, h));
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_s
     if (rec) {
       v3 p, nn;
       float u, v;
-      hit_record(S, h, o, d, time, &p, &nn, &u, &v);
+      hit_record<MEDIA>(S, h, o, d, rc, &p, &nn, &u, &v);
       const float vals[8] = {u, v, p.x, p.y, p.z, nn.x, nn.y, nn.z};
 #pragma unroll
       for (int k = 0; k < 8; ++k) rec[k * n + i] = vals[k];
@@ -110,6 +111,7 @@ struct tpt_scene {
   std::vector<uint8_t> sph;       // per scene: every leaf is a sphere
   std::vector<uint8_t> simple;    // per scene: constant textures, no light or one xz_rect light
   std::vector<uint8_t> motion;    // per scene: has moving spheres (rays need their time)
+  std::vector<uint8_t> media;     // per scene: has constant_medium (queries need media draws)
   std::vector<uint32_t> n_slots_of;
   std::vector<uint32_t> n_leaves_of;
   scene_dev *d_scenes = nullptr;
@@ -253,6 +255,18 @@ int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t g
 }
 
 // Every index the kernels follow must stay inside its array.
+// A medium's boundary: entries in range, none of them a medium (the device
+// evaluates a boundary without media).
+bool medium_ok(const tpt_flat_scene *f, const float *w) {
+  uint32_t first, count;
+  std::memcpy(&first, &w[0], 4);
+  std::memcpy(&count, &w[1], 4);
+  if (count == 0 || static_cast<uint64_t>(first) + count > f->n_entries) return false;
+  for (uint32_t e = first; e < first + count; ++e)
+    if (f->entries[e].b == TPT_SLOT_MEDIUM) return false;
+  return true;
+}
+
 int validate_flat(const tpt_flat_scene *f) {
   if (!f) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: null scene");
   if (f->n_segments > 0 && (!f->slots || !f->segments))
@@ -274,13 +288,15 @@ int validate_flat(const tpt_flat_scene *f) {
       std::memcpy(&cnt, &s.f[1], 4);
       if (static_cast<uint64_t>(s.a) + cnt > f->n_entries)
         return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: list entries out of range");
-    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT || kind == TPT_SLOT_MSPHERE) {
+    } else if (kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_RECT || kind == TPT_SLOT_MSPHERE ||
+               kind == TPT_SLOT_MEDIUM) {
       if ((s.a & TPT_PAYLOAD_MASK) >= f->n_materials)
         return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: material index out of range");
       uint32_t mi;
       std::memcpy(&mi, &s.f[0], 4);
       if (kind == TPT_SLOT_MSPHERE && (!f->motions || mi >= f->n_motions))
         return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: motion index out of range");
+      if (kind == TPT_SLOT_MEDIUM && !medium_ok(f, s.f)) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad medium");
     } else {
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: unknown slot kind");
     }
@@ -290,8 +306,9 @@ int validate_flat(const tpt_flat_scene *f) {
     uint32_t mi;
     std::memcpy(&mi, &en.f[0], 4);
     if ((en.a & TPT_PAYLOAD_MASK) >= f->n_materials || (en.xform != 0xFFFFFFFFu && en.xform >= f->n_xforms) ||
-        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT && en.b != TPT_SLOT_MSPHERE) ||
-        (en.b == TPT_SLOT_MSPHERE && (!f->motions || mi >= f->n_motions)))
+        (en.b != TPT_SLOT_SPHERE && en.b != TPT_SLOT_RECT && en.b != TPT_SLOT_MSPHERE && en.b != TPT_SLOT_MEDIUM) ||
+        (en.b == TPT_SLOT_MSPHERE && (!f->motions || mi >= f->n_motions)) ||
+        (en.b == TPT_SLOT_MEDIUM && (en.xform != 0xFFFFFFFFu || !medium_ok(f, en.f))))
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad list entry");
   }
   for (uint32_t x = 0; x < f->n_xforms; ++x)
@@ -300,7 +317,9 @@ int validate_flat(const tpt_flat_scene *f) {
     const tpt_material &M = f->materials[m];
     if ((M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT) && M.tex >= f->n_textures)
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: texture index out of range");
-    if (M.kind > TPT_MAT_DIFFUSE_LIGHT) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad material");
+    if ((M.kind == TPT_MAT_ISOTROPIC) && M.tex >= f->n_textures)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: texture index out of range");
+    if (M.kind > TPT_MAT_ISOTROPIC) return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: bad material");
   }
   for (uint32_t t = 0; t < f->n_textures; ++t) {
     const tpt_texture &T = f->textures[t];
@@ -355,7 +374,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   std::vector<tpt_light> lights;
   std::vector<tpt_motion> motions;
   std::vector<scene_dev> scn(n);
-  std::vector<uint8_t> sweep(n), sph(n), simple(n), motion(n);
+  std::vector<uint8_t> sweep(n), sph(n), simple(n), motion(n), media(n);
   std::vector<uint32_t> nslots(n), nleaves(n);
   for (uint32_t k = 0; k < n; ++k) {
     const tpt_flat_scene &f = scenes[k];
@@ -379,6 +398,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       } else {
         s.a = rebase_word(s.a, static_cast<uint32_t>(mb));
         if (kind == TPT_SLOT_MSPHERE) s.f[0] = rebase_bits(s.f[0], static_cast<uint32_t>(mob));
+        if (kind == TPT_SLOT_MEDIUM) s.f[0] = rebase_bits(s.f[0], static_cast<uint32_t>(eb));
       }
       slots.push_back(s);
     }
@@ -388,12 +408,14 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       if (en.xform != 0xFFFFFFFFu) en.xform += static_cast<uint32_t>(xb);
       en.a = rebase_word(en.a, static_cast<uint32_t>(mb));
       if (en.b == TPT_SLOT_MSPHERE) en.f[0] = rebase_bits(en.f[0], static_cast<uint32_t>(mob));
+      if (en.b == TPT_SLOT_MEDIUM) en.f[0] = rebase_bits(en.f[0], static_cast<uint32_t>(eb));
       entries.push_back(en);
     }
     xforms.insert(xforms.end(), f.xforms, f.xforms + f.n_xforms);
     for (uint32_t m = 0; m < f.n_materials; ++m) {
       tpt_material M = f.materials[m];
-      if (M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT) M.tex += static_cast<uint32_t>(tb);
+      if (M.kind == TPT_MAT_LAMBERTIAN || M.kind == TPT_MAT_DIFFUSE_LIGHT || M.kind == TPT_MAT_ISOTROPIC)
+        M.tex += static_cast<uint32_t>(tb);
       materials.push_back(M);
     }
     for (uint32_t t = 0; t < f.n_textures; ++t) {
@@ -416,11 +438,15 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     d.light_mode = f.n_lights ? f.light_mode : 0;
     for (int q = 0; q < 6; ++q) d.bounds[q] = f.bounds[q];
     motion[k] = f.n_motions > 0;
+    media[k] = 0;
+    for (uint32_t i = 0; i < f.n_slots; ++i) media[k] |= (f.slots[i].b >> 28) == TPT_SLOT_MEDIUM;
+    for (uint32_t e = 0; e < f.n_entries; ++e) media[k] |= f.entries[e].b == TPT_SLOT_MEDIUM;
     // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
     // close to each lane's own count only when the whole tree is small.  (The
-    // sweep kernel is built without the ray-time replay moving spheres need.)
-    sweep[k] = f.n_slots <= 64 && !motion[k];
-    simple[k] = !motion[k] && (f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT));
+    // sweep kernel is built without the ray-time replay and the media draws.)
+    sweep[k] = f.n_slots <= 64 && !motion[k] && !media[k];
+    simple[k] = !motion[k] && !media[k] &&
+                (f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT));
     for (uint32_t t = 0; t < f.n_textures; ++t) simple[k] = simple[k] && f.textures[t].kind == TPT_TEX_CONSTANT;
     nslots[k] = f.n_slots;
     nleaves[k] = f.n_leaves;
@@ -454,6 +480,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   s->sph = sph;
   s->simple = simple;
   s->motion = motion;
+  s->media = media;
   s->n_slots_of = nslots;
   s->n_leaves_of = nleaves;
   if (hipHostMalloc(reinterpret_cast<void **>(&s->host_count), NSUB * CSTRIDE * sizeof(uint32_t), hipHostMallocDefault) !=
@@ -550,11 +577,15 @@ int tpt_trace_closest_timed(tpt_scene *s, const float *d_rays, const float *d_ti
   if (blocks > 0x7FFFFFFFull) return tpt::fail(TPT_ERR_INVALID, "tpt_trace_closest: too many rays");
   const scene_view S = s->view(0);
   if (s->sweep[0])
-    hipLaunchKernelGGL(k_trace_closest<true>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+    hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
+                       d_leaf, d_t, d_rec, d_mat);
+  else if (s->media[0])
+    hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
                        d_leaf, d_t, d_rec, d_mat);
   else
-    hipLaunchKernelGGL(k_trace_closest<false>, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+    hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
                        d_leaf, d_t, d_rec, d_mat);
   HIP_TRY(hipGetLastError());
@@ -641,6 +672,12 @@ void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
   if (B.sweep)
     hipLaunchKernelGGL((k_trace<true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+  else if (B.multi && B.P.media)
+    hipLaunchKernelGGL((k_trace<false, true, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                       parity);
+  else if (B.P.media)
+    hipLaunchKernelGGL((k_trace<false, false, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                       parity);
   else if (B.multi && B.sph)
     hipLaunchKernelGGL((k_trace<false, true, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
   else if (B.multi)
@@ -676,7 +713,8 @@ void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
 int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch &L, tpt_guide *guide,
                 tpt_render_stats *stats, void *stream_ptr) {
   if (!s || (n_jobs && !jobs)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
-  if (L.flags & ~TPT_RENDER_PROFILE) return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
+  if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER))
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
   if (L.queue_capacity > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
   if (L.blocks_per_cu > 64) return tpt::fail(TPT_ERR_INVALID, "tpt_render: blocks_per_cu above 64");
@@ -771,6 +809,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     B.sph = B.sph && s->sph[jobs[j].scene];
     B.simple = B.simple && s->simple[jobs[j].scene];
     if (s->motion[jobs[j].scene]) B.P.motion = 1;
+    if (s->media[jobs[j].scene]) B.P.media = 1;
   }
   B.multi = multi;
   B.guided = guide != nullptr;
@@ -780,6 +819,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   P.hdr = s->d_hdr;
   P.pass = s->d_pass;
   P.guide_verts = ~0u;
+  P.media_scatter = (L.flags & TPT_RENDER_MEDIA_SCATTER) ? 1u : 0u;
   if (guide) {
     P.g_tiles = guide->xtiles;
     P.g_params = guide->params;

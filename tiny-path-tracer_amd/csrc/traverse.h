@@ -34,6 +34,7 @@
 namespace tptd {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr float kFltMaxT = 3.402823466e38f;  // std::numeric_limits<float>::max()
 
 struct hit_t {
   float t;
@@ -45,6 +46,14 @@ struct hit_t {
 struct slot_t {
   float f[6];
   uint32_t a, b;
+};
+
+// Per-query context: the ray's time (moving spheres) and the Philox counter
+// words and key of its media draws (a medium with leaf id L draws from
+// counter (c0, c1, c2, 2^20 + L)).
+struct ray_ctx {
+  float time;
+  uint32_t c0, c1, c2, k0, k1;
 };
 
 __device__ __forceinline__ slot_t load_slot(const tpt_slot *p) {
@@ -146,6 +155,12 @@ __device__ __forceinline__ bool msphere_t(const scene_view &S, float idx_bits, v
   return sphere_t(c.x, c.y, c.z, m.radius, o, d, tmin, tmax, t);
 }
 
+// The draws of medium `leaf` for this query: word 0 the free path, 1-3 the
+// arbitrary normal of a hit.
+__device__ __forceinline__ uint4 medium_draws(const ray_ctx &rc, uint32_t leaf) {
+  return philox(rc.c0, rc.c1, rc.c2, (1u << 20) + leaf, rc.k0, rc.k1);
+}
+
 // f = (k, a0, a1, b0, b1); axis = normal axis; in-plane axes (i0, i1) are
 // (1,2) for yz, (0,2) for xz, (0,1) for xy.
 __device__ __forceinline__ bool rect_t(const float *f, uint32_t axis, v3 o, v3 d, float tmin, float tmax,
@@ -192,11 +207,44 @@ __device__ __forceinline__ void to_world(const tpt_xform &x, v3 *p, v3 *n) {
   }
 }
 
-// LIST leaf: primitives in order, t_max shrinking to each accepted t.
-__device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 o, v3 d, float time, float tmin,
-                                       float tmax, float *t, uint32_t *entry, uint32_t *mat) {
-  const uint32_t first = s.a;
-  const uint32_t count = __float_as_uint(s.f[1]);
+template <bool MEDIA>
+__device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, uint32_t count, uint32_t leaf0, v3 o,
+                                          v3 d, const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
+                                          uint32_t *mat);
+
+// constant_medium::hit (src/hitable.cc:98-128): the boundary's first hit over
+// the whole line, its next hit from there + 0.0001, clipped to (t_min, t_max);
+// the medium stops the ray at a free path of -ln(u)/density along it.
+__device__ __forceinline__ bool medium_t(const scene_view &S, const float *f, uint32_t leaf, v3 o, v3 d,
+                                         const ray_ctx &rc, float tmin, float tmax, float *t) {
+  const uint32_t first = __float_as_uint(f[0]), count = __float_as_uint(f[1]);
+  float t1, t2;
+  uint32_t e, m;
+  if (!list_eval<false>(S, first, count, 0, o, d, rc, -kFltMaxT, kFltMaxT, &t1, &e, &m)) return false;
+  const float tmin2 = static_cast<float>(static_cast<double>(t1) + 0.0001);
+  if (!list_eval<false>(S, first, count, 0, o, d, rc, tmin2, kFltMaxT, &t2, &e, &m)) return false;
+  if (t1 < tmin) t1 = tmin;
+  if (t2 > tmax) t2 = tmax;
+  if (t1 >= t2) return false;
+  const float len = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  const float inside = (t2 - t1) * len;
+  const uint4 w = medium_draws(rc, leaf);
+  const float hit_distance =
+      static_cast<float>(static_cast<double>(-1.0f / f[2]) * log(static_cast<double>(u01(w.x))));
+  if (hit_distance < inside) {
+    *t = t1 + hit_distance / len;
+    return true;
+  }
+  return false;
+}
+
+// LIST leaf: primitives in order, t_max shrinking to each accepted t (a
+// medium's boundary is evaluated the same way, MEDIA = false).
+template <bool MEDIA>
+__device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, uint32_t count, uint32_t leaf0, v3 o,
+                                          v3 d, const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
+                                          uint32_t *mat) {
+  const float time = rc.time;
   float cur = tmax;
   bool found = false;
   uint32_t cached = kNone;
@@ -210,9 +258,13 @@ __device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 
       cached = en.xform;
     }
     float th;
-    const bool h = en.b == TPT_SLOT_SPHERE    ? sphere_t(en.f[0], en.f[1], en.f[2], en.f[3], lo, ld, tmin, cur, &th)
-                   : en.b == TPT_SLOT_MSPHERE ? msphere_t(S, en.f[0], lo, ld, time, tmin, cur, &th)
-                                              : rect_t(en.f, (en.a >> 28) & 3u, lo, ld, tmin, cur, &th);
+    bool h;
+    if (MEDIA && en.b == TPT_SLOT_MEDIUM)
+      h = medium_t(S, en.f, leaf0 + (e - first), lo, ld, rc, tmin, cur, &th);
+    else
+      h = en.b == TPT_SLOT_SPHERE    ? sphere_t(en.f[0], en.f[1], en.f[2], en.f[3], lo, ld, tmin, cur, &th)
+          : en.b == TPT_SLOT_MSPHERE ? msphere_t(S, en.f[0], lo, ld, time, tmin, cur, &th)
+                                     : rect_t(en.f, (en.a >> 28) & 3u, lo, ld, tmin, cur, &th);
     if (h) {
       cur = th;
       found = true;
@@ -225,9 +277,11 @@ __device__ __forceinline__ bool list_t(const scene_view &S, const slot_t &s, v3 
 }
 
 // SPH: the scene's leaves are all spheres (no rect or list code is compiled).
-template <bool SPH = false>
-__device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d, float time,
-                                       float tmin, float tmax, float *t, uint32_t *entry, uint32_t *mat) {
+// MEDIA: the scene may hold constant_media (their code is compiled only then).
+template <bool SPH = false, bool MEDIA = false>
+__device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d,
+                                       const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
+                                       uint32_t *mat) {
   if (SPH || kind == TPT_SLOT_SPHERE) {
     *entry = kNone;
     *mat = s.a;
@@ -241,17 +295,23 @@ __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uin
   if (kind == TPT_SLOT_MSPHERE) {
     *entry = kNone;
     *mat = s.a;
-    return msphere_t(S, s.f[0], o, d, time, tmin, tmax, t);
+    return msphere_t(S, s.f[0], o, d, rc.time, tmin, tmax, t);
   }
-  return list_t(S, s, o, d, time, tmin, tmax, t, entry, mat);
+  if (MEDIA && kind == TPT_SLOT_MEDIUM) {
+    *entry = kNone;
+    *mat = s.a;
+    return medium_t(S, s.f, s.b & TPT_PAYLOAD_MASK, o, d, rc, tmin, tmax, t);
+  }
+  return list_eval<MEDIA>(S, s.a, __float_as_uint(s.f[1]), __float_as_uint(s.f[0]), o, d, rc, tmin, tmax, t, entry,
+                         mat);
 }
 
 // Exact replay of the bvh_node recursion for one segment: evaluates each node
 // with a fresh result, combining (left, right) as `left.t < right.t ? left :
 // right`.  Stack entries: a pending right child, or a finished left result.
-template <bool SPH = false>
-__device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv, float time,
-                                                        float tmin, float tmax, hit_t *out) {
+template <bool SPH = false, bool MEDIA = false>
+__device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t start, v3 o, v3 d, v3 inv,
+                                                        const ray_ctx &rc, float tmin, float tmax, hit_t *out) {
   struct frame {
     uint32_t tag;  // bit31 set: finished-left result; else pending right slot
     uint32_t slot, entry, mat;
@@ -278,7 +338,7 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
       } else {
         float t;
         uint32_t entry, mat;
-        have = leaf_t<SPH>(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat);
+        have = leaf_t<SPH, MEDIA>(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat);
         acc = hit_t{t, i, entry, mat};
       }
       break;
@@ -311,9 +371,9 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
 
 // Fast pre-order walk of one segment (t_min, t_max not NaN).  *saw_nan is set
 // when a leaf returned a NaN t (the caller then uses segment_exact).
-template <bool SPH = false>
+template <bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
-                                             float time, float tmin, float tmax, hit_t *out, bool *saw_nan,
+                                             const ray_ctx &rc, float tmin, float tmax, hit_t *out, bool *saw_nan,
                                              uint32_t *nodes) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
@@ -329,7 +389,7 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
     } else {
       float t;
       uint32_t entry, mat;
-      if (leaf_t<SPH>(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat)) {
+      if (leaf_t<SPH, MEDIA>(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat)) {
         if (t != t) *saw_nan = true;
         if (!have || t <= best.t) {
           best = hit_t{t, i, entry, mat};
@@ -363,8 +423,8 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // slot kind does not diverge.  Lanes with `live` false take no part.
 // Must be called by all lanes of the wave together.
 __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t start, uint32_t end, bool live, v3 o, v3 d,
-                                              v3 inv, float time, float tmin, float tmax, hit_t *out, bool *saw_nan,
-                                              uint32_t *nodes) {
+                                              v3 inv, const ray_ctx &rc, float tmin, float tmax, hit_t *out,
+                                              bool *saw_nan, uint32_t *nodes) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t next = live ? start : end;
@@ -381,7 +441,7 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
       } else {
         float t;
         uint32_t entry, mat;
-        if (leaf_t(S, s, kind, o, d, time, tmin, tmax, &t, &entry, &mat)) {
+        if (leaf_t(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat)) {
           if (t != t) *saw_nan = true;
           if (!have || t <= best.t) {
             best = hit_t{t, i, entry, mat};
@@ -406,8 +466,8 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
 // NaN bound (a caller's t_min, or t_max after a segment whose winner had a
 // NaN t) sends the segment to segment_exact, which keeps the reference's slab
 // updates verbatim.
-template <bool SWEEP = false, bool SPH = false>
-__device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, float time, float tmin, float tmax,
+template <bool SWEEP = false, bool SPH = false, bool MEDIA = false>
+__device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, const ray_ctx &rc, float tmin, float tmax,
                                             hit_t *out, uint32_t *nodes, bool live = true) {
   const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
   float cur = tmax;
@@ -420,12 +480,12 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, flo
     const bool bounded = tmin == tmin && cur == cur;
     bool got = false;
     if (SWEEP) {
-      got = segment_sweep(S, start, end, live && bounded, o, d, inv, time, tmin, cur, &h, &saw_nan, nodes);
+      got = segment_sweep(S, start, end, live && bounded, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes);
     } else {
       if (!live) continue;
-      if (bounded) got = segment_fast<SPH>(S, start, end, o, d, inv, time, tmin, cur, &h, &saw_nan, nodes);
+      if (bounded) got = segment_fast<SPH, MEDIA>(S, start, end, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes);
     }
-    if (live && (saw_nan || !bounded)) got = segment_exact<SPH>(S, start, o, d, inv, time, tmin, cur, &h);
+    if (live && (saw_nan || !bounded)) got = segment_exact<SPH, MEDIA>(S, start, o, d, inv, rc, tmin, cur, &h);
     if (got) {
       *out = h;
       cur = h.t;
@@ -443,8 +503,12 @@ __device__ __forceinline__ uint32_t leaf_id(const scene_view &S, const hit_t &h)
 }
 
 // Full hit_record of the winner: point, normal (flip applied), u, v.
-__device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, v3 o, v3 d, float time, v3 *p, v3 *n,
-                                           float *u, float *v) {
+// MEDIUM_NORMAL: draw a medium hit's arbitrary normal as constant_medium::hit
+// does; the shading kernels skip it (the isotropic phase function ignores the
+// normal) and report (0, 1, 0).
+template <bool MEDIUM_NORMAL = false>
+__device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, v3 o, v3 d, const ray_ctx &rc, v3 *p,
+                                           v3 *n, float *u, float *v) {
   const slot_t s = load_slot(&S.slots[h.slot]);
   uint32_t kind = s.b >> 28;
   float f[5] = {s.f[0], s.f[1], s.f[2], s.f[3], s.f[4]};
@@ -462,11 +526,26 @@ __device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, 
   const float t = h.t;
   v3 pt{lo.x + t * ld.x, lo.y + t * ld.y, lo.z + t * ld.z};
   v3 nn;
+  if (kind == TPT_SLOT_MEDIUM) {
+    // constant_medium::hit: an arbitrary normal vec3(u, u, u).normalize()
+    // (src/hitable.cc:121-122); u and v are not set there (0 here).
+    *p = pt;
+    *n = v3{0.0f, 1.0f, 0.0f};
+    if (MEDIUM_NORMAL) {
+      const uint4 w = medium_draws(rc, leaf_id(S, h));
+      nn = v3{u01(w.y), u01(w.z), u01(w.w)};
+      const float k = static_cast<float>(1.0 / static_cast<double>(sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z)));
+      *n = v3{nn.x * k, nn.y * k, nn.z * k};
+    }
+    *u = 0.0f;
+    *v = 0.0f;
+    return;
+  }
   if (kind == TPT_SLOT_MSPHERE) {
     // moving_sphere::hit (src/sphere.cc:53-57): normal about the centre at
     // the ray's time, uv about center0.
     const tpt_motion m = S.motions[__float_as_uint(f[0])];
-    const v3 c = motion_center(m, time);
+    const v3 c = motion_center(m, rc.time);
     const float r = m.radius;
     const v3 q{(pt.x - m.center0[0]) / r, (pt.y - m.center0[1]) / r, (pt.z - m.center0[2]) / r};
     *u = static_cast<float>(static_cast<double>(atan2f(q.z, q.x)) / (2 * M_PI)) + 0.5f;

@@ -99,6 +99,9 @@ class flattener {
 
  private:
   flat_scene *out_;
+  bool in_boundary_ = false;     // collecting a medium's boundary
+  bool skip_material_ = false;   // boundary primitives number no material
+  std::vector<prim_ref> pending_boundary_;  // set by prim_words for a medium
   std::map<std::vector<xop>, uint32_t> xform_ids_;
   std::map<const material *, uint32_t> material_ids_;
   std::map<const texture *, uint32_t> texture_ids_;
@@ -130,6 +133,9 @@ class flattener {
       s.f[3 + i] = bn->box_.max_.e[i];
     }
     out_->slots.push_back(s);
+    if (bn->right_ == bn->left_ && dynamic_cast<const constant_medium *>(bn->left_))
+      // the reference evaluates that child twice, each time with its own draw
+      throw flatten_error(TPT_ERR_UNSUPPORTED, "constant_medium as the only child of a bvh_node");
     child(bn->left_, depth);
     uint32_t right = TPT_NO_RIGHT;
     if (bn->right_ != bn->left_) {  // n == 1 stores one child twice; it is evaluated once
@@ -175,7 +181,10 @@ class flattener {
         throw flatten_error(TPT_ERR_UNSUPPORTED, "more than 4 nested translate/rotate_y wrappers");
       refs->push_back(prim_ref{h, chain, flip});
     } else if (dynamic_cast<const constant_medium *>(h)) {
-      throw flatten_error(TPT_ERR_UNSUPPORTED, "constant_medium is not on the device path");
+      if (!chain.empty() || flip)
+        throw flatten_error(TPT_ERR_UNSUPPORTED, "constant_medium below translate/rotate_y/flip_normal");
+      if (in_boundary_) throw flatten_error(TPT_ERR_UNSUPPORTED, "constant_medium inside a medium's boundary");
+      refs->push_back(prim_ref{h, chain, flip});
     } else {
       throw flatten_error(TPT_ERR_UNSUPPORTED, "unknown hitable type in scene graph");
     }
@@ -192,6 +201,21 @@ class flattener {
       f[4] = 0.0f;
       mat = material_id(s->mat_ptr_);
       *kind = TPT_SLOT_SPHERE;
+    } else if (auto *cm = dynamic_cast<const constant_medium *>(h)) {
+      // boundary primitives are appended by emit_leaf; f[0] (first entry) is
+      // patched there, f[1] = count
+      mat = material_id(cm->phase_funcion_);
+      in_boundary_ = true;
+      std::vector<prim_ref> b;
+      collect(cm->boundary_, {}, 0, &b);
+      in_boundary_ = false;
+      if (b.empty()) throw flatten_error(TPT_ERR_INVALID, "constant_medium with an empty boundary");
+      f[0] = 0.0f;
+      f[1] = bitsf(static_cast<uint32_t>(b.size()));
+      f[2] = cm->density_;
+      f[3] = f[4] = 0.0f;
+      pending_boundary_ = std::move(b);
+      *kind = TPT_SLOT_MEDIUM;
     } else if (auto *ms = dynamic_cast<const moving_sphere *>(h)) {
       tpt_motion m{};
       for (int i = 0; i < 3; ++i) m.center0[i] = ms->center0_.e[i], m.center1[i] = ms->center1_.e[i];
@@ -223,6 +247,23 @@ class flattener {
     *a = mat | (axis << 28) | (flip << 30);
   }
 
+  // A medium boundary's primitives: only their geometry is used (the medium
+  // hit takes its material from the phase function), so no material is
+  // numbered for them.
+  void boundary_entries(const std::vector<prim_ref> &refs) {
+    for (const prim_ref &r : refs) {
+      tpt_entry e{};
+      uint32_t kind, a;
+      skip_material_ = true;
+      prim_words(r.prim, r.flip, e.f, &a, &kind);
+      skip_material_ = false;
+      e.a = a;
+      e.b = kind;
+      e.xform = r.chain.empty() ? 0xFFFFFFFFu : xform_id(r.chain);
+      out_->entries.push_back(e);
+    }
+  }
+
   void emit_leaf(const std::vector<prim_ref> &refs) {
     const uint32_t leaf = out_->n_leaves;
     if (leaf + refs.size() > TPT_PAYLOAD_MASK) throw flatten_error(TPT_ERR_UNSUPPORTED, "too many leaves");
@@ -231,6 +272,10 @@ class flattener {
       float f[5];
       uint32_t a, kind;
       prim_words(refs[0].prim, refs[0].flip, f, &a, &kind);
+      if (kind == TPT_SLOT_MEDIUM) {
+        f[0] = bitsf(static_cast<uint32_t>(out_->entries.size()));
+        boundary_entries(pending_boundary_);
+      }
       for (int i = 0; i < 5; ++i) s.f[i] = f[i];
       s.a = a;
       s.b = (kind << 28) | leaf;
@@ -240,13 +285,20 @@ class flattener {
       s.f[0] = bitsf(leaf);
       s.f[1] = bitsf(static_cast<uint32_t>(refs.size()));
       s.b = TPT_SLOT_LIST << 28;
+      // the list's entries are contiguous; media boundaries follow them
+      std::vector<std::pair<size_t, std::vector<prim_ref>>> media;
       for (const prim_ref &r : refs) {
         tpt_entry e{};
         uint32_t kind;
         prim_words(r.prim, r.flip, e.f, &e.a, &kind);
         e.b = kind;
         e.xform = r.chain.empty() ? 0xFFFFFFFFu : xform_id(r.chain);
+        if (kind == TPT_SLOT_MEDIUM) media.emplace_back(out_->entries.size(), std::move(pending_boundary_));
         out_->entries.push_back(e);
+      }
+      for (auto &m : media) {
+        out_->entries[m.first].f[0] = bitsf(static_cast<uint32_t>(out_->entries.size()));
+        boundary_entries(m.second);
       }
       out_->n_leaves += static_cast<uint32_t>(refs.size());
     }
@@ -269,6 +321,7 @@ class flattener {
   }
 
   uint32_t material_id(const material *m) {
+    if (skip_material_) return 0;
     if (!m) throw flatten_error(TPT_ERR_INVALID, "scene primitive with a null material");
     auto it = material_ids_.find(m);
     if (it != material_ids_.end()) return it->second;
@@ -286,6 +339,9 @@ class flattener {
     } else if (auto *dl = dynamic_cast<const diffuse_light *>(m)) {
       rec.kind = TPT_MAT_DIFFUSE_LIGHT;
       rec.tex = texture_id(dl->emit_);
+    } else if (auto *iso = dynamic_cast<const isotropic *>(m)) {
+      rec.kind = TPT_MAT_ISOTROPIC;
+      rec.tex = texture_id(iso->albedo_);
     } else {
       throw flatten_error(TPT_ERR_UNSUPPORTED, "material type not on the device path");
     }

@@ -24,7 +24,7 @@ namespace tpt {
 const char *scene_names() {
   return "random_spheres,random_spheres_textured,cornell555,random_spheres_n<N>,ref_cornell_box,"
          "ref_sphere_cornell_box,ref_two_perlin_spheres,ref_light_spheres,ref_two_checker_spheres,box_list,"
-         "ref_random_scene";
+         "ref_random_scene,cornell_smoke";
 }
 
 std::vector<unsigned char> noise_image(int width, int height, uint64_t seed) {
@@ -126,6 +126,14 @@ bool build_now(const std::string &name, uint32_t seed, scene_spec *s, std::strin
     s->background = 1.0f;
     const vec3 from(13, 2, 3), at(0, 0, 0);
     make_camera(s, from, at, 20.0f, 0.0f, (from - at).length(), 0.0f, 1.0f);
+    return true;
+  }
+  if (name == "cornell_smoke") {
+    s->world = tpt_scene_cornell_smoke(&s->lights);
+    s->width = s->height = 400;
+    s->spp = 64;
+    s->max_depth = 50;
+    make_camera(s, vec3(278, 278, -800), vec3(278, 278, 0), 40.0f, 0.0f, 10.0f);
     return true;
   }
   if (name == "ref_light_spheres") {
