@@ -203,7 +203,14 @@ typedef struct tpt_flat_scene {
   float bounds[6];    /* world bounding_box (min xyz, max xyz); normalises guiding inputs */
   const tpt_motion *motions; /* moving spheres (MSPHERE slots and entries index it) */
   uint32_t n_motions;
+  /* One word per segment, or NULL: a segment that is a bvh_node seen through
+   * translate / rotate_y / flip_normal wrappers (headers/rect_box.h:46-113)
+   * holds its xform index | flip<<31; TPT_SEGMENT_PLAIN for the others. */
+  const uint32_t *segment_xforms;
 } tpt_flat_scene;
+
+#define TPT_SEGMENT_PLAIN 0x7FFFFFFFu
+#define TPT_SEGMENT_FLIP 0x80000000u
 
 /* Camera fields as computed by camera_with_blur's constructor
  * (src/camera.cc:2-21); ray generation follows get_ray (src/camera.cc:23-31). */
@@ -247,6 +254,12 @@ int tpt_host_scene_flat(const tpt_host_scene *hs, tpt_flat_scene *flat);
 int tpt_host_scene_camera(const tpt_host_scene *hs, tpt_camera *camera);
 int tpt_host_scene_info(const tpt_host_scene *hs, tpt_scene_info *info);
 int tpt_host_scene_destroy(tpt_host_scene *hs);
+
+/* load_image_texture (src/utils.cc:236-240, stb_image there): decodes a
+ * baseline JPEG or an 8-bit PPM (P6/P3) into *out (width*height*3 bytes of
+ * RGB, rows top first; free with tpt_image_free). */
+int tpt_image_load(const char *path, uint8_t **out, int *width, int *height);
+void tpt_image_free(uint8_t *pixels);
 
 /* The fields of camera_with_blur(lookfrom, lookat, vup, vfov, aspect,
  * aperture, focus_dist, t0, t1) as its constructor computes them

@@ -143,6 +143,15 @@ bool build(const std::string &name, ref_scene_t *s, hitable **world) {
     *world = tpt_scene_ref_random_scene();
   } else if (name == "cornell_smoke") {
     *world = tpt_scene_cornell_smoke(&lights);
+  } else if (name == "ref_oneweek_final" || name == "ref_earth") {
+    // stb_image through the reference's own loader, from the working directory
+    int w = 0, h = 0, c = 0;
+    unsigned char *px = load_image_texture("earthmap.jpg", w, h, c);
+    if (!px || c != 3) return false;
+    s->image.assign(px, px + static_cast<size_t>(w) * h * 3);
+    free(px);
+    *world = name == "ref_earth" ? tpt_scene_ref_earth(s->image.data(), w, h)
+                                 : tpt_scene_ref_oneweek_final(s->image.data(), w, h, &lights);
   } else if (name == "native:cornell_box") {
     // The reference's own factory and its driver's light list (main.cpp:99-106).
     *world = cornell_box();
@@ -256,6 +265,17 @@ int ref_trace(void *sp, const float *rays, uint64_t n, float tmin, float tmax, i
   for (int k = 1; k < nthreads; ++k) ts.emplace_back(worker);
   worker();
   for (auto &th : ts) th.join();
+  return 0;
+}
+
+// The reference's load_image_texture (src/utils.cc:236-240, stb_image):
+// copies at most cap bytes of the decoded image into out.
+int ref_load_image(const char *path, unsigned char *out, long cap, int *w, int *h, int *c) {
+  unsigned char *data = load_image_texture(std::string(path), *w, *h, *c);
+  if (!data) return -1;
+  const long n = static_cast<long>(*w) * *h * *c;
+  std::memcpy(out, data, static_cast<size_t>(n < cap ? n : cap));
+  free(data);
   return 0;
 }
 

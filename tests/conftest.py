@@ -48,3 +48,20 @@ def device(tpt):
     if tpt.lib().tpt_device_count() < 1 or not torch.cuda.is_available():
         pytest.fail("GPU test without a visible gfx950 device")
     return torch.device("cuda:0")
+
+
+@pytest.fixture
+def earth_cwd(tmp_path, monkeypatch):
+    """A working directory holding an earthmap.jpg (the scenes that use the
+    earth texture load it from there, as the reference does): a seeded
+    1024x512 baseline JPEG written with PIL."""
+    import numpy as np
+
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(2024)
+    y, x = np.mgrid[0:512, 0:1024] / 512.0
+    img = np.stack([127 + 100 * np.sin(3 * x + c) * np.cos(5 * y - c) for c in (0.0, 1.0, 2.0)], -1)
+    img = np.clip(img + rng.normal(0, 8, img.shape), 0, 255).astype(np.uint8)
+    Image.fromarray(img).save(tmp_path / "earthmap.jpg", "JPEG", quality=90, subsampling=0)
+    monkeypatch.chdir(tmp_path)
+    return tmp_path

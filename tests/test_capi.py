@@ -138,6 +138,28 @@ def test_media_flatten(tpt, oracle):
     assert list(mk) == [0, 0, 3, 0, 4, 4]  # green, red, light, white, two isotropic phase functions
 
 
+def test_wrapped_bvh_segment_flatten(tpt, oracle, earth_cwd):
+    # ref_oneweek_final: the 1000 spheres' bvh_node under rotate_y(15) and
+    # translate is its own segment with a 2-op wrapper chain
+    import ctypes as C
+
+    hs = tpt.HostScene("ref_oneweek_final")
+    f = hs.flat
+    assert f.n_leaves == oracle.scene("ref_oneweek_final").n_leaves == 3409
+    assert f.n_segments == 3 and f.n_motions == 1 and f.segment_xforms
+    words = (C.c_uint32 * 3).from_address(f.segment_xforms)
+    assert words[0] == 0x7FFFFFFF and words[1] == 0x7FFFFFFF and words[2] < f.n_xforms
+    xf = (C.c_uint32 * 16).from_address(f.xforms + 16 * 4 * words[2])  # tpt_xform: nops, kinds, p[4][3], pad
+    assert xf[0] == 2 and xf[1] == 0 | (1 << 2)  # translate (outermost), then rotate_y
+    assert tpt.HostScene("ref_earth").flat.n_leaves == 1
+
+
+def test_earth_scene_needs_its_image(tpt, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(tpt.TptError, match="earthmap.jpg"):
+        tpt.HostScene("ref_oneweek_final")
+
+
 def test_unsupported_types_fail_cleanly(tpt):
     # the product scene library has no unsupported scene; the flattener's
     # refusal is exercised through a scene name that does not exist

@@ -17,10 +17,10 @@ def cam_for(tpt, name):
     return tpt.HostScene(name).camera
 
 
-def gpu_render(tpt, name, w, h, spp, seed, max_depth=None, media_scatter=False):
+def gpu_render(tpt, name, w, h, spp, seed, max_depth=None, media_scatter=False, background=None):
     sc = tpt.Scene.named(name)
     s, sq, st = sc.render(width=w, height=h, spp=spp, seed=seed, max_depth=max_depth, camera=sc.host.camera,
-                          media_scatter=media_scatter)
+                          media_scatter=media_scatter, background=background)
     return s.cpu().numpy(), sq.cpu().numpy(), st
 
 
@@ -117,3 +117,21 @@ def test_media_scatter_mode_matches_oracle(tpt, oracle, device):
     assert np.mean(np.abs(z) > 4) < 0.01
     s_b, _, _ = gpu_render(tpt, "cornell_smoke", w, w, spp, seed=21)
     assert s_g.mean() > 1.02 * s_b.mean()
+
+
+@pytest.mark.parametrize("name,w,spp", [("ref_oneweek_final", 32, 16), ("ref_earth", 32, 8)])
+def test_earth_scenes_match_oracle(tpt, oracle, device, earth_cwd, name, w, spp):
+    """Scenes with the earth texture (decoded by this build from
+    earthmap.jpg in the working directory): same streams as the oracle.  The
+    final scene's light is one-sided and faces up (the reference's
+    diffuse_light), so a sky on misses (the background extension) lights it."""
+    s_g, q_g, st = gpu_render(tpt, name, w, w, spp, seed=13, background=1.0)
+    s_c, q_c = oracle.scene(name).render(w, w, spp, seed=13, background=1.0)
+    assert s_c.mean() > 0.01
+    assert st["paths"] == w * w * spp
+    close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
+    frac = close.all(axis=2).mean()
+    print(f"{name}: {frac:.4f} of pixels equal to rounding")
+    assert frac > 0.85
+    z = zscores(s_g, q_g, spp, s_c, q_c, spp)
+    assert np.mean(np.abs(z) > 4) < 0.02

@@ -114,3 +114,18 @@ def test_trace_moving_spheres(tpt, oracle, device):
     # without times the rays see every sphere at time 0 (the ray default)
     rays = s.make_rays(0, 1 << 14, seed=5)
     compare(s, sc, rays, torch, device, record=False)
+
+
+def test_trace_wrapped_bvh_media_and_earth(tpt, oracle, device, earth_cwd):
+    """The reference's oneweek_final: a bvh_node under rotate_y and translate
+    (a transformed segment), media inside the top-level list, a moving sphere
+    and the earth texture; timed rays, bit for bit with the oracle."""
+    import torch
+
+    s = oracle.scene("ref_oneweek_final")
+    sc = tpt.Scene.named("ref_oneweek_final")
+    rng = np.random.default_rng(11)
+    for kind in range(4):
+        rays = s.make_rays(kind, 1 << 15, seed=401 + kind)
+        time = rng.uniform(0, 1, rays.shape[1]).astype(np.float32)
+        compare(s, sc, rays, torch, device, time=time)

@@ -95,3 +95,40 @@ def test_media_match_reference(oracle, reference):
             na, nb = (a_leaf == m).sum(), (b_leaf == m).sum()
             # two binomial counts of the same (unknown) rate
             assert abs(na - nb) <= 4 * np.sqrt(na + nb + 1), (kind, m, na, nb)
+
+
+def test_oneweek_final_matches_reference(oracle, reference, tmp_path, monkeypatch):
+    """The reference's oneweek_final (src/utils.cc:358-414) built by the shared
+    scene text against both APIs (the reference decoding earthmap.jpg with its
+    stb_image, the oracle with this build's decoder): 400 boxes in a BVH, a
+    moving sphere, media, the earth texture and 1000 spheres in a bvh_node
+    under rotate_y and translate.  Rays that no medium decides on either side
+    agree bit for bit (leaf, t, record); a NaN t (in-plane box faces) lets a
+    medium's random stop change the later list elements' t_max, so those rays
+    are left to the statistical comparison."""
+    import os
+    import shutil
+
+    src = "/root/reference/resources/earthmap.jpg"
+    if not os.path.exists(src):
+        pytest.skip("the reference tree is not here")
+    shutil.copy(src, tmp_path / "earthmap.jpg")
+    monkeypatch.chdir(tmp_path)
+    s = oracle.scene("ref_oneweek_final")
+    h = reference.scene("ref_oneweek_final")
+    assert s.n_leaves == reference.leaf_count(h) == 3409
+    media = [2405, 2406]  # the list after the 2400 box faces: light, moving sphere, glass, metal, ball, 2 media
+    rng = np.random.default_rng(9)
+    for kind in range(4):
+        rays = s.make_rays(kind, 1 << 14, seed=5 + kind)
+        time = rng.uniform(0, 1, rays.shape[1]).astype(np.float32)
+        a = s.trace(rays, record=True, time=time)
+        b = reference.trace(h, rays, record=True, time=time)
+        calm = ~np.isin(a[0], media) & ~np.isin(b[0], media) & ~np.isnan(a[1]) & ~np.isnan(b[1])
+        assert calm.mean() > 0.5
+        assert np.array_equal(a[0][calm], b[0][calm]), f"kind {kind}"
+        assert np.array_equal(a[1][calm].view(np.uint32), b[1][calm].view(np.uint32)), f"kind {kind}"
+        assert np.array_equal(a[2][:, calm].view(np.uint32), b[2][:, calm].view(np.uint32)), f"kind {kind}"
+        for m in media:
+            na, nb = (a[0] == m).sum(), (b[0] == m).sum()
+            assert abs(na - nb) <= 4 * np.sqrt(na + nb + 1), (kind, m, na, nb)

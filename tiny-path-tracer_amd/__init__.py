@@ -48,7 +48,7 @@ class FlatScene(C.Structure):
         ("perlin", C.c_void_p),
         ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("light_mode", C.c_uint32),
         ("n_leaves", C.c_uint32), ("max_depth", C.c_uint32), ("bounds", C.c_float * 6),
-        ("motions", C.c_void_p), ("n_motions", C.c_uint32),
+        ("motions", C.c_void_p), ("n_motions", C.c_uint32), ("segment_xforms", C.c_void_p),
     ]
 
 

@@ -48,6 +48,7 @@ struct scene_view {
   const tpt_perlin *perlin;
   const tpt_light *lights;
   const tpt_motion *motions;
+  const uint32_t *seg_xf;  // per segment: xform | flip<<31 (TPT_SEGMENT_PLAIN), or null: none wrapped
   uint32_t n_slots, n_segments, n_lights, light_mode;
 };
 

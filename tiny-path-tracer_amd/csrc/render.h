@@ -153,6 +153,7 @@ __device__ __forceinline__ scene_view view_of(const scene_view &S, const render_
   const scene_dev &sc = P.scenes[P.jobs[job].scene];
   scene_view v = S;
   v.segments = S.segments + 2 * sc.seg_begin;
+  v.seg_xf = S.seg_xf ? S.seg_xf + sc.seg_begin : nullptr;
   v.n_segments = sc.n_seg;
   v.lights = S.lights + sc.light_begin;
   v.n_lights = sc.n_lights;
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
 // in as separate restrict arguments so that wave-uniform loads from them can
 // use the scalar data path.
 template <bool SWEEP, bool MULTI, bool SPH = false, bool MEDIA = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : 1))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : (MEDIA ? 1 : 5)))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
   static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");

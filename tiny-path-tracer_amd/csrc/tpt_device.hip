@@ -104,6 +104,7 @@ struct tpt_scene {
   tpt_perlin *perlin = nullptr;
   tpt_light *lights = nullptr;
   tpt_motion *motions = nullptr;
+  uint32_t *seg_xf = nullptr;  // per pooled segment; null when no scene has a wrapped segment
   uint32_t n_slots = 0, n_segments = 0, n_lights = 0;
   // the pool's scenes
   std::vector<scene_dev> scn;
@@ -149,6 +150,7 @@ struct tpt_scene {
     v.perlin = perlin;
     v.lights = lights + scn[k].light_begin;
     v.motions = motions;
+    v.seg_xf = seg_xf ? seg_xf + scn[k].seg_begin : nullptr;
     v.n_slots = n_slots;
     v.n_segments = scn[k].n_seg;
     v.n_lights = scn[k].n_lights;
@@ -159,6 +161,7 @@ struct tpt_scene {
   scene_view pool_view() const {
     scene_view v = view(0);
     v.segments = segments;
+    v.seg_xf = seg_xf;
     v.n_segments = n_segments;
     v.lights = lights;
     v.n_lights = n_lights;
@@ -179,6 +182,7 @@ void free_scene(tpt_scene *s) {
   (void)hipFree(s->perlin);
   (void)hipFree(s->lights);
   (void)hipFree(s->motions);
+  (void)hipFree(s->seg_xf);
   (void)hipFree(s->d_scenes);
   (void)hipFree(s->state);
   (void)hipFree(s->d_jobs);
@@ -275,6 +279,11 @@ int validate_flat(const tpt_flat_scene *f) {
   for (uint32_t g = 0; g < f->n_segments; ++g) {
     if (f->segments[2 * g] > f->segments[2 * g + 1] || f->segments[2 * g + 1] > f->n_slots)
       return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: segment out of range");
+    if (f->segment_xforms) {
+      const uint32_t x = f->segment_xforms[g] & TPT_SEGMENT_PLAIN;
+      if (x != TPT_SEGMENT_PLAIN && x >= f->n_xforms)
+        return tpt::fail(TPT_ERR_INVALID, "tpt_scene_create: segment xform out of range");
+    }
   }
   for (uint32_t i = 0; i < f->n_slots; ++i) {
     const tpt_slot &s = f->slots[i];
@@ -373,6 +382,8 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
   std::vector<tpt_perlin> perlin;
   std::vector<tpt_light> lights;
   std::vector<tpt_motion> motions;
+  std::vector<uint32_t> seg_xf;
+  bool any_wrapped = false;
   std::vector<scene_dev> scn(n);
   std::vector<uint8_t> sweep(n), sph(n), simple(n), motion(n), media(n);
   std::vector<uint32_t> nslots(n), nleaves(n);
@@ -403,6 +414,12 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       slots.push_back(s);
     }
     for (uint32_t g = 0; g < 2 * f.n_segments; ++g) segments.push_back(f.segments[g] + static_cast<uint32_t>(sb));
+    for (uint32_t g = 0; g < f.n_segments; ++g) {
+      uint32_t w = f.segment_xforms ? f.segment_xforms[g] : TPT_SEGMENT_PLAIN;
+      if ((w & TPT_SEGMENT_PLAIN) != TPT_SEGMENT_PLAIN) w = (w & TPT_SEGMENT_FLIP) | ((w & TPT_SEGMENT_PLAIN) + static_cast<uint32_t>(xb));
+      any_wrapped = any_wrapped || w != TPT_SEGMENT_PLAIN;
+      seg_xf.push_back(w);
+    }
     for (uint32_t e = 0; e < f.n_entries; ++e) {
       tpt_entry en = f.entries[e];
       if (en.xform != 0xFFFFFFFFu) en.xform += static_cast<uint32_t>(xb);
@@ -444,7 +461,10 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     // Wave-uniform sweeps pay for the union of the lanes' visits; that stays
     // close to each lane's own count only when the whole tree is small.  (The
     // sweep kernel is built without the ray-time replay and the media draws.)
-    sweep[k] = f.n_slots <= 64 && !motion[k] && !media[k];
+    bool wrapped = false;
+    for (uint32_t g = 0; f.segment_xforms && g < f.n_segments; ++g) wrapped |= f.segment_xforms[g] != TPT_SEGMENT_PLAIN;
+    sweep[k] = f.n_slots <= 64 && !motion[k] && !media[k] && !wrapped;
+    if (wrapped) sph[k] = 0;  // the sphere-only walk is built without segment transforms
     simple[k] = !motion[k] && !media[k] &&
                 (f.n_lights == 0 || (f.light_mode == 0 && f.lights[0].kind == TPT_LIGHT_XZ_RECT));
     for (uint32_t t = 0; t < f.n_textures; ++t) simple[k] = simple[k] && f.textures[t].kind == TPT_TEX_CONSTANT;
@@ -467,6 +487,7 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
       (rc = upload(&s->perlin, perlin.data(), perlin.size())) ||
       (rc = upload(&s->lights, lights.data(), lights.size())) ||
       (rc = upload(&s->motions, motions.data(), motions.size())) ||
+      (rc = upload(&s->seg_xf, seg_xf.data(), any_wrapped ? seg_xf.size() : 0)) ||
       (rc = upload(&s->d_scenes, scn.data(), scn.size()))) {
     free_scene(s);
     delete s;

@@ -469,7 +469,8 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
 template <bool SWEEP = false, bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, const ray_ctx &rc, float tmin, float tmax,
                                             hit_t *out, uint32_t *nodes, bool live = true) {
-  const v3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const v3 inv0{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  const v3 o0 = o, d0 = d;
   float cur = tmax;
   bool any = false;
   for (uint32_t g = 0; g < S.n_segments; ++g) {
@@ -477,6 +478,16 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, con
     hit_t h;
     bool saw_nan = false;
     if (start >= end) continue;
+    // a bvh_node seen through translate / rotate_y: its walk takes the ray
+    // those wrappers hand down (headers/rect_box.h:87-95, src/rect_box.cc:171-180);
+    // scenes with such segments use neither the sweep nor the sphere-only walk
+    o = o0, d = d0;
+    v3 inv = inv0;
+    const uint32_t sx = !SWEEP && !SPH && S.seg_xf ? S.seg_xf[g] : TPT_SEGMENT_PLAIN;
+    if ((sx & TPT_SEGMENT_PLAIN) != TPT_SEGMENT_PLAIN) {
+      to_local(S.xforms[sx & TPT_SEGMENT_PLAIN], &o, &d);
+      inv = v3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    }
     const bool bounded = tmin == tmin && cur == cur;
     bool got = false;
     if (SWEEP) {
@@ -515,6 +526,12 @@ __device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, 
   uint32_t word = s.a;
   v3 lo = o, ld = d;
   uint32_t xf = kNone;
+  // the wrappers of the winner's segment, if any (outermost, applied first)
+  uint32_t sx = TPT_SEGMENT_PLAIN;
+  if (S.seg_xf)
+    for (uint32_t g = 0; g < S.n_segments; ++g)
+      if (h.slot >= S.segments[2 * g] && h.slot < S.segments[2 * g + 1]) sx = S.seg_xf[g];
+  if ((sx & TPT_SEGMENT_PLAIN) != TPT_SEGMENT_PLAIN) to_local(S.xforms[sx & TPT_SEGMENT_PLAIN], &lo, &ld);
   if (kind == TPT_SLOT_LIST) {
     const tpt_entry en = S.entries[h.entry];
     for (int k = 0; k < 5; ++k) f[k] = en.f[k];
@@ -529,19 +546,16 @@ __device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, 
   if (kind == TPT_SLOT_MEDIUM) {
     // constant_medium::hit: an arbitrary normal vec3(u, u, u).normalize()
     // (src/hitable.cc:121-122); u and v are not set there (0 here).
-    *p = pt;
-    *n = v3{0.0f, 1.0f, 0.0f};
+    nn = v3{0.0f, 1.0f, 0.0f};
     if (MEDIUM_NORMAL) {
       const uint4 w = medium_draws(rc, leaf_id(S, h));
       nn = v3{u01(w.y), u01(w.z), u01(w.w)};
       const float k = static_cast<float>(1.0 / static_cast<double>(sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z)));
-      *n = v3{nn.x * k, nn.y * k, nn.z * k};
+      nn = v3{nn.x * k, nn.y * k, nn.z * k};
     }
     *u = 0.0f;
     *v = 0.0f;
-    return;
-  }
-  if (kind == TPT_SLOT_MSPHERE) {
+  } else if (kind == TPT_SLOT_MSPHERE) {
     // moving_sphere::hit (src/sphere.cc:53-57): normal about the centre at
     // the ray's time, uv about center0.
     const tpt_motion m = S.motions[__float_as_uint(f[0])];
@@ -568,6 +582,8 @@ __device__ __forceinline__ void hit_record(const scene_view &S, const hit_t &h, 
   }
   if ((word >> 30) & 1u) nn = -nn;
   if (xf != kNone) to_world(S.xforms[xf], &pt, &nn);
+  if (sx & TPT_SEGMENT_FLIP) nn = -nn;
+  if ((sx & TPT_SEGMENT_PLAIN) != TPT_SEGMENT_PLAIN) to_world(S.xforms[sx & TPT_SEGMENT_PLAIN], &pt, &nn);
   *p = pt;
   *n = nn;
 }

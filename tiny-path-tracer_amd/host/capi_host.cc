@@ -1,12 +1,15 @@
 // Host half of the C ABI (include/tpt.h): error reporting and the scene
 // library.  The device half lives in csrc/tpt_device.hip.
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/tpt.h"
 #include "errors.h"
 #include "flatten.h"
+#include "image_load.h"
 #include "scenes.h"
 
 namespace {
@@ -88,6 +91,20 @@ int tpt_camera_make(const float lookfrom[3], const float lookat[3], const float 
   *out = tpt::camera_fields(cam);
   return TPT_OK;
 }
+
+int tpt_image_load(const char *path, uint8_t **out, int *width, int *height) {
+  if (!path || !out || !width || !height) return tpt::fail(TPT_ERR_INVALID, "tpt_image_load: null argument");
+  *out = nullptr;
+  std::vector<uint8_t> rgb;
+  std::string err;
+  if (!tpt::load_image_rgb8(path, &rgb, width, height, &err)) return tpt::fail(TPT_ERR_INVALID, err);
+  *out = static_cast<uint8_t *>(std::malloc(rgb.size()));
+  if (!*out) return tpt::fail(TPT_ERR_INVALID, "tpt_image_load: out of memory");
+  std::memcpy(*out, rgb.data(), rgb.size());
+  return TPT_OK;
+}
+
+void tpt_image_free(uint8_t *pixels) { std::free(pixels); }
 
 int tpt_host_scene_destroy(tpt_host_scene *hs) {
   // The object graph uses raw new and is never freed, like the reference's

@@ -55,35 +55,49 @@ class flattener {
 
   void world(const hitable *w) {
     if (!w) throw flatten_error(TPT_ERR_INVALID, "world is null");
+    std::vector<const hitable *> elems;
     if (dynamic_cast<const hitable_list *>(w) && !dynamic_cast<const bvh_node *>(w)) {
       // Top-level list: nested lists flatten into one sequence (the shrinking
-      // t_max threads through them unchanged); bvh_node elements become their
-      // own segments, runs of other elements one LIST leaf segment.
-      std::vector<const hitable *> elems;
+      // t_max threads through them unchanged); bvh_node elements (bare or
+      // seen through translate / rotate_y / flip_normal) become their own
+      // segments, runs of other elements one LIST leaf segment.
       expand_list(w, &elems);
-      size_t i = 0;
-      while (i < elems.size()) {
-        const uint32_t start = static_cast<uint32_t>(out_->slots.size());
-        if (dynamic_cast<const bvh_node *>(elems[i])) {
-          node(elems[i], 1);
-          ++i;
-        } else {
-          std::vector<prim_ref> refs;
-          while (i < elems.size() && !dynamic_cast<const bvh_node *>(elems[i])) {
-            collect(elems[i], {}, 0, &refs);
-            ++i;
-          }
-          emit_leaf(refs);
-        }
-        out_->segments.push_back(start);
-        out_->segments.push_back(static_cast<uint32_t>(out_->slots.size()));
-      }
     } else {
+      elems.push_back(w);
+    }
+    size_t i = 0;
+    while (i < elems.size()) {
       const uint32_t start = static_cast<uint32_t>(out_->slots.size());
-      node(w, dynamic_cast<const bvh_node *>(w) ? 1 : 0);
+      std::vector<xop> chain;
+      uint32_t flip = 0;
+      const hitable *inner = peel(elems[i], &chain, &flip);
+      uint32_t word = TPT_SEGMENT_PLAIN;
+      if (dynamic_cast<const bvh_node *>(inner)) {
+        if (!chain.empty() || flip) {
+          if (chain.size() > TPT_XFORM_MAX_OPS)
+            throw flatten_error(TPT_ERR_UNSUPPORTED, "more than 4 nested translate/rotate_y wrappers");
+          word = (chain.empty() ? TPT_SEGMENT_PLAIN : xform_id(chain)) | (flip ? TPT_SEGMENT_FLIP : 0u);
+        }
+        node(inner, 1);
+        ++i;
+      } else if (elems.size() == 1) {
+        node(elems[i], 0);
+        ++i;
+      } else {
+        std::vector<prim_ref> refs;
+        while (i < elems.size() && !dynamic_cast<const bvh_node *>(peel(elems[i], nullptr, nullptr))) {
+          collect(elems[i], {}, 0, &refs);
+          ++i;
+        }
+        emit_leaf(refs);
+      }
       out_->segments.push_back(start);
       out_->segments.push_back(static_cast<uint32_t>(out_->slots.size()));
+      out_->segment_xforms.push_back(word);
     }
+    bool wrapped = false;
+    for (uint32_t x : out_->segment_xforms) wrapped = wrapped || x != TPT_SEGMENT_PLAIN;
+    if (!wrapped) out_->segment_xforms.clear();
   }
 
   void lights(const hitable *l) {
@@ -105,6 +119,25 @@ class flattener {
   std::map<std::vector<xop>, uint32_t> xform_ids_;
   std::map<const material *, uint32_t> material_ids_;
   std::map<const texture *, uint32_t> texture_ids_;
+
+  // The hitable under a chain of translate / rotate_y / flip_normal wrappers
+  // (chain outermost first; null outputs are not filled).
+  static const hitable *peel(const hitable *h, std::vector<xop> *chain, uint32_t *flip) {
+    for (;;) {
+      if (auto *f = dynamic_cast<const flip_normal *>(h)) {
+        if (flip) *flip ^= 1u;
+        h = f->ptr_;
+      } else if (auto *t = dynamic_cast<const translate *>(h)) {
+        if (chain) chain->push_back(xop{TPT_XFORM_TRANSLATE, {t->offset_.e[0], t->offset_.e[1], t->offset_.e[2]}});
+        h = t->ptr_;
+      } else if (auto *r = dynamic_cast<const rotate_y *>(h)) {
+        if (chain) chain->push_back(xop{TPT_XFORM_ROTATE_Y, {r->sin_theta_, r->cos_theta_, 0.0f}});
+        h = r->ptr_;
+      } else {
+        return h;
+      }
+    }
+  }
 
   static void expand_list(const hitable *h, std::vector<const hitable *> *out) {
     const auto *lst = dynamic_cast<const hitable_list *>(h);
@@ -453,6 +486,7 @@ tpt_flat_scene flat_scene::view() const {
   for (int i = 0; i < 6; ++i) v.bounds[i] = bounds[i];
   v.motions = motions.data();
   v.n_motions = static_cast<uint32_t>(motions.size());
+  v.segment_xforms = segment_xforms.empty() ? nullptr : segment_xforms.data();
   return v;
 }
 

@@ -34,6 +34,7 @@ struct flat_scene {
   std::vector<tpt_perlin> perlin;  // 0 or 1 element
   std::vector<tpt_light> lights;
   std::vector<tpt_motion> motions;
+  std::vector<uint32_t> segment_xforms;  // per segment; empty when no segment is wrapped
   uint32_t light_mode = 0;
   uint32_t n_leaves = 0;
   uint32_t max_depth = 0;

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <limits>
 #include <random>
@@ -24,6 +25,7 @@
 #include "../scene/sphere.h"
 #include "../scene/texture.h"
 #include "../scene/utils.h"
+#include "image_load.h"
 
 // ---------------------------------------------------------------- drand_r ---
 namespace {
@@ -235,37 +237,19 @@ perlin_noise::perlin_noise() {
 }
 
 // ------------------------------------------------------ load_image_texture ---
+// src/utils.cc:236-240 returns stbi_load's buffer (rows top first, `channels`
+// bytes per pixel); image_texture reads 3 bytes per pixel.  This build's own
+// decoder (host/image_load.cc) always returns 3 channels (gray replicated).
 unsigned char *load_image_texture(std::string filename, int &width, int &height, int &channels) {
-  std::ifstream in(filename, std::ios::binary);
-  if (!in) return nullptr;
-  std::string magic;
-  in >> magic;
-  if (magic != "P6") return nullptr;
-  int maxval = 0;
-  auto skip_comments = [&]() {
-    in >> std::ws;
-    while (in.peek() == '#') {
-      std::string line;
-      std::getline(in, line);
-      in >> std::ws;
-    }
-  };
-  skip_comments();
-  in >> width;
-  skip_comments();
-  in >> height;
-  skip_comments();
-  in >> maxval;
-  in.get();
-  if (!in || width <= 0 || height <= 0 || maxval != 255) return nullptr;
-  channels = 3;
-  const size_t n = static_cast<size_t>(width) * height * 3;
-  auto *data = static_cast<unsigned char *>(std::malloc(n));
+  std::vector<uint8_t> rgb;
+  std::string err;
+  int w = 0, h = 0;
+  if (!tpt::load_image_rgb8(filename, &rgb, &w, &h, &err)) return nullptr;
+  auto *data = static_cast<unsigned char *>(std::malloc(rgb.size()));
   if (!data) return nullptr;
-  in.read(reinterpret_cast<char *>(data), static_cast<std::streamsize>(n));
-  if (static_cast<size_t>(in.gcount()) != n) {
-    std::free(data);
-    return nullptr;
-  }
+  std::memcpy(data, rgb.data(), rgb.size());
+  width = w;
+  height = h;
+  channels = 3;
   return data;
 }

@@ -24,7 +24,7 @@ namespace tpt {
 const char *scene_names() {
   return "random_spheres,random_spheres_textured,cornell555,random_spheres_n<N>,ref_cornell_box,"
          "ref_sphere_cornell_box,ref_two_perlin_spheres,ref_light_spheres,ref_two_checker_spheres,box_list,"
-         "ref_random_scene,cornell_smoke";
+         "ref_random_scene,cornell_smoke,ref_oneweek_final,ref_earth";
 }
 
 std::vector<unsigned char> noise_image(int width, int height, uint64_t seed) {
@@ -126,6 +126,34 @@ bool build_now(const std::string &name, uint32_t seed, scene_spec *s, std::strin
     s->background = 1.0f;
     const vec3 from(13, 2, 3), at(0, 0, 0);
     make_camera(s, from, at, 20.0f, 0.0f, (from - at).length(), 0.0f, 1.0f);
+    return true;
+  }
+  if (name == "ref_oneweek_final" || name == "ref_earth") {
+    // the earth map from the working directory, as the reference's
+    // stbi_load("earthmap.jpg") / load_image_texture find it
+    int w = 0, h = 0, c = 0;
+    unsigned char *px = load_image_texture("earthmap.jpg", w, h, c);
+    if (!px) {
+      *err = name + ": earthmap.jpg (baseline JPEG or PPM) not found or not decodable in the working directory";
+      return false;
+    }
+    s->image.assign(px, px + static_cast<size_t>(w) * h * 3);
+    std::free(px);
+    if (name == "ref_earth") {
+      s->world = tpt_scene_ref_earth(s->image.data(), w, h);
+      s->width = s->height = 400;
+      s->spp = 64;
+      s->max_depth = 50;
+      s->background = 1.0f;
+      const vec3 from(13, 2, 3), at(0, 0, 0);
+      make_camera(s, from, at, 20.0f, 0.0f, (from - at).length());
+      return true;
+    }
+    s->world = tpt_scene_ref_oneweek_final(s->image.data(), w, h, &s->lights);
+    s->width = s->height = 400;
+    s->spp = 64;
+    s->max_depth = 50;
+    make_camera(s, vec3(478, 278, -600), vec3(278, 278, 0), 40.0f, 0.0f, 10.0f, 0.0f, 1.0f);
     return true;
   }
   if (name == "cornell_smoke") {

@@ -19,8 +19,9 @@ inline float float_min(float a, float b) { return a < b ? a : b; }
 inline float float_max(float a, float b) { return a > b ? a : b; }
 
 // Loads an 8-bit image for image_texture.  The reference uses stb_image
-// (src/utils.cc:236-240); this build reads binary PPM (P6) and returns nullptr
-// for other formats.  The buffer is malloc'ed and owned by the caller.
+// (src/utils.cc:236-240); this build decodes baseline JPEG and PPM (P6, P3) to
+// RGB (channels = 3) and returns nullptr for other files.  The buffer is
+// malloc'ed and owned by the caller.
 unsigned char *load_image_texture(std::string filename, int &width, int &height, int &channels);
 
 namespace tpt {
