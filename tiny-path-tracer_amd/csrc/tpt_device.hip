@@ -627,7 +627,12 @@ int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min
 
 namespace {
 
-constexpr uint64_t kPassItems = 1ull << 26;  // res buffer: 768 MB of per-sample results
+// Items per pass.  A pass ends by draining the queues (its last bounces run
+// at low occupancy), so passes are long: 2^28 items hold 3.2 GB of per-sample
+// results.  With training rows recorded a pass is 2^26 items (the rows of one
+// pass are kept until the next).
+constexpr uint64_t kPassItems = 1ull << 28;
+constexpr uint64_t kPassItemsRecording = 1ull << 26;
 constexpr uint32_t kMaxEnt = 0xFFFFu;        // pass entry index shares a word with the depth
 
 struct pass_plan {
@@ -636,8 +641,8 @@ struct pass_plan {
 };
 
 // The jobs' sample layers, job after job, packed into passes of at most
-// kPassItems items and kMaxEnt entries (a job has one entry per pass).
-std::vector<pass_plan> plan_passes(const tpt_job *jobs, uint32_t n_jobs) {
+// pass_items items and kMaxEnt entries (a job has one entry per pass).
+std::vector<pass_plan> plan_passes(const tpt_job *jobs, uint32_t n_jobs, uint64_t pass_items) {
   std::vector<pass_plan> out;
   pass_plan cur;
   for (uint32_t j = 0; j < n_jobs; ++j) {
@@ -645,7 +650,7 @@ std::vector<pass_plan> plan_passes(const tpt_job *jobs, uint32_t n_jobs) {
     uint32_t s = jobs[j].sample_offset;
     const uint32_t s_end = jobs[j].sample_offset + jobs[j].spp;
     while (s < s_end) {
-      const uint64_t room = (kPassItems - cur.n_items) / npix;
+      const uint64_t room = (pass_items - cur.n_items) / npix;
       if (room == 0 || cur.ents.size() >= kMaxEnt) {
         out.push_back(cur);
         cur = pass_plan();
@@ -746,7 +751,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     const tpt_job &J = jobs[j];
     if (J.scene >= s->scn.size()) return tpt::fail(TPT_ERR_INVALID, "tpt_render: job scene out of range");
     if (J.width == 0 || J.height == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: empty image");
-    if (static_cast<uint64_t>(J.width) * J.height > kPassItems)
+    if (static_cast<uint64_t>(J.width) * J.height > kPassItemsRecording)
       return tpt::fail(TPT_ERR_INVALID, "tpt_render: more than 2^26 pixels");
     if (!J.d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null d_sum");
     if (J.max_depth > 0xFFFEu) return tpt::fail(TPT_ERR_INVALID, "tpt_render: max_depth above 65534");
@@ -779,7 +784,8 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     d.sum = J.d_sum;
     d.sumsq = J.d_sumsq;
   }
-  const std::vector<pass_plan> plan = plan_passes(jobs, n_jobs);
+  const std::vector<pass_plan> plan =
+      plan_passes(jobs, n_jobs, guide && L.guiding ? kPassItemsRecording : kPassItems);
   uint32_t max_items = 0, max_ent = 0;
   for (const auto &pp : plan) {
     max_items = std::max(max_items, pp.n_items);
@@ -813,7 +819,9 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   const uint32_t grid = static_cast<uint32_t>(s->n_cu) * bpc;
   // k_guide_shade holds 51 KB of LDS (weights + output table): 3 blocks per CU
   const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * 3;
-  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 21);
+  // 2^24 records in flight by default (4.8 GB of queues): launches long enough
+  // that their ramp-up and tail cost little
+  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 24);
   cap = std::min(cap, max_items);
   if ((rc = ensure_state(s, cap, max_items, grid, guide_grid)) != TPT_OK) return rc;
 
