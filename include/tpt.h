@@ -399,7 +399,7 @@ int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, cons
                     tpt_guide *guide, tpt_render_stats *stats, void *stream);
 
 /* Autotuner: times calibration renders of the jobs (at most calib_spp
- * samples each, 0 = 2, into scratch sums) over queue capacities (2^20..2^23),
+ * samples each, 0 = 2, into scratch sums) over queue capacities (2^21..2^24),
  * grid sizes (4, 8, 16 blocks per CU) and graph replay lengths (0, 8), and
  * writes the fastest settings into *best (its flags, guiding and
  * guide_vertices are kept).  *best_ms: that calibration's wall time. */

@@ -1093,7 +1093,7 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
     cj[j].d_sum = scratch[j];
     cj[j].d_sumsq = nullptr;
   }
-  const uint32_t caps[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+  const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
   const uint32_t bpcs[] = {4, 8, 16};
   const uint32_t graphs[] = {0, 8};
   tpt_launch base = *best;
