@@ -411,7 +411,12 @@ def main():
         "detail": {"Mrays_per_s": round(segs / dt / 1e6, 2), "segments_per_path": round(segs / max(paths, 1), 3),
                    "light_queries": totals["light_queries"], "nodes_per_segment":
                        round(totals["nodes_visited"] / max(segs, 1), 2), "trace_ms": round(totals["trace_ms"], 1),
-                   "shade_ms": round(totals["shade_ms"], 1), "nan_samples": totals["nan_samples"]},
+                   "shade_ms": round(totals["shade_ms"], 1), "nan_samples": totals["nan_samples"],
+                   # BVH / primitive bytes the walk requests: one 32-byte slot per visit (boxes and
+                   # leaves alike), served from L1/L2 (the Cornell box's slots are 576 bytes)
+                   "bvh_slot_bytes_per_segment": round(32 * totals["nodes_visited"] / max(segs, 1), 1),
+                   "bvh_requested_GBps": round(32 * totals["nodes_visited"] / (totals["trace_ms"] / 1e3) / 1e9, 1)
+                   if totals["trace_ms"] else None},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.scene, W, H, args.cpu_spp, depth)
