@@ -149,3 +149,25 @@ def test_bad_jobs_are_rejected(tpt, pool, hosts):
     jobs, _ = batch_jobs(pool, specs(hosts)[:1])
     with pytest.raises(tpt.TptError, match="guiding needs"):
         pool.render_jobs(jobs, launch=L)
+
+
+def test_pool_with_motion_media_and_wrapped_bvh(tpt, earth_cwd):
+    """Moving spheres, media and a bvh_node under wrappers in one pool: each
+    job equals its own single render bit for bit."""
+    import torch
+
+    names = ["cornell_smoke", "ref_random_scene", "ref_oneweek_final", "cornell555"]
+    hosts = [tpt.HostScene(n) for n in names]
+    pool = tpt.Scene.pool(hosts)
+    sps = [dict(scene=k, width=24 + 8 * k, height=20 + 4 * k, spp=[4, 3, 2, 5][k], seed=31 + k, sample_offset=k,
+                max_depth=50) for k in range(4)]
+    jobs, bufs = batch_jobs(pool, sps)
+    L = tpt.Launch()
+    L.graph_iters = 8
+    pool.render_jobs(jobs, launch=L)
+    for sp, (s, q) in zip(sps, bufs):
+        h = hosts[sp["scene"]]
+        single = tpt.Scene(h)
+        s1, q1, _ = single.render(width=sp["width"], height=sp["height"], spp=sp["spp"], seed=sp["seed"],
+                                  sample_offset=sp["sample_offset"], max_depth=sp["max_depth"], camera=h.camera)
+        assert torch.equal(s, s1) and torch.equal(q, q1), names[sp["scene"]]

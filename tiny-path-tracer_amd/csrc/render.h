@@ -585,7 +585,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
       } else {
         v3 pt, n;
         float u, v;
-        hit_record(S, h, p.o, p.d,
+        // (the job's view: a wrapped segment is looked up among its scene's segments only)
+        hit_record(Sj, h, p.o, p.d,
                    !SIMPLE && (P.motion | P.media) ? make_ray_ctx(P, c, p.depth) : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u},
                    &pt, &n, &u, &v);
         const tpt_material M = S.materials[h.mat];
@@ -739,7 +740,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       float u, v;
       // (the diffuse queue holds lambertian hits only: no media)
       const ray_ctx rc{P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f, 0u, 0u, 0u, 0u, 0u};
-      hit_record(S, h, p.o, p.d, rc, &pt, &n, &u, &v);
+      hit_record(view_of<MULTI>(S, P, MULTI ? P.pass[p.ent].job : 0u), h, p.o, p.d, rc, &pt, &n, &u, &v);
       const v3 wo = -unit(p.d);
       const float pv[3] = {pt.x, pt.y, pt.z};
       const float *bounds = P.scenes[P.jobs[MULTI ? P.pass[p.ent].job : 0u].scene].bounds;
@@ -786,16 +787,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const float4 hw = Q.mq[Q_DIFFUSE][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
+      const scene_view Sj = view_of<MULTI>(S, P, c.job);
       v3 pt, n;
       float u, v;
-      hit_record(S, h, p.o, p.d, ray_ctx{P.motion ? ray_time(P, c) : 0.0f, 0u, 0u, 0u, 0u, 0u}, &pt, &n, &u, &v);
+      hit_record(Sj, h, p.o, p.d, ray_ctx{P.motion ? ray_time(P, c) : 0.0f, 0u, 0u, 0u, 0u, 0u}, &pt, &n, &u, &v);
       vmf_mix mix;
       const bool finite = decode_mix_bf16(reinterpret_cast<const uint16_t *>(outs + threadIdx.x * 10), &mix);
       mix_norms(&mix);
       // a non-finite network output falls back to the reference sampler at this vertex
       const float alpha = finite ? alpha_net : 0.0f;
       const tpt_material M = S.materials[h.mat];
-      const scene_view Sj = view_of<MULTI>(S, P, c.job);
       const bool lights = Sj.n_lights > 0;
       stream R;
       R.init(c.px, c.smp, p.depth + 1, P.jobs[c.job].seed);
