@@ -702,7 +702,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
 // weighted atten (cos/π) q / (p_ref m), p_ref being the reference's mixture
 // value (cosine_pdf::value is cos θ without 1/π), so the expectation is the
 // reference estimator's for any a and g.
-template <bool MULTI>
+//
+// SIMPLE: as for k_shade (constant textures, at most one rect light, no
+// moving spheres).
+template <bool MULTI, bool SIMPLE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
   __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
@@ -726,20 +729,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < total;
-    // Phase 1: encoding inputs of this lane's entry (only its queue position
-    // stays live across the network passes; phase 2 reloads the entry).
-    uint32_t off = 0;
+    // Phase 1: the entry's hit record and encoding inputs.  The hit point,
+    // (u, v) and material stay live across the network passes (the normal is
+    // in9[3..5]); phase 2 reloads only the path record.
+    uint32_t off = 0, hmat = 0;
     float in9[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 1.0f};
+    v3 pt{0.0f, 0.0f, 0.0f};
+    float u = 0.0f, v = 0.0f;
     if (valid) {
       const uint32_t r = find_region(pre, NSUB, qi);
       off = r * Q.region + (qi - pre[r]);
       const path p = load_ray(Q.mq[Q_DIFFUSE], off);
       const float4 hw = Q.mq[Q_DIFFUSE][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
-      v3 pt, n;
-      float u, v;
-      // (the diffuse queue holds lambertian hits only: no media)
-      const ray_ctx rc{P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f, 0u, 0u, 0u, 0u, 0u};
+      hmat = h.mat;
+      v3 n;
+      // (the diffuse queue holds lambertian hits only: no media; the job's
+      // view: a wrapped segment is looked up among its scene's segments only)
+      const ray_ctx rc{!SIMPLE && P.motion ? ray_time(P, item_in_entry(P, MULTI ? p.ent : 0u, p.item)) : 0.0f, 0u, 0u, 0u, 0u, 0u};
       hit_record(view_of<MULTI>(S, P, MULTI ? P.pass[p.ent].job : 0u), h, p.o, p.d, rc, &pt, &n, &u, &v);
       const v3 wo = -unit(p.d);
       const float pv[3] = {pt.x, pt.y, pt.z};
@@ -784,37 +791,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     path p{};
     if (valid) {
       p = load_ray(Q.mq[Q_DIFFUSE], off);
-      const float4 hw = Q.mq[Q_DIFFUSE][3][off];
-      const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
       const scene_view Sj = view_of<MULTI>(S, P, c.job);
-      v3 pt, n;
-      float u, v;
-      hit_record(Sj, h, p.o, p.d, ray_ctx{P.motion ? ray_time(P, c) : 0.0f, 0u, 0u, 0u, 0u, 0u}, &pt, &n, &u, &v);
+      const v3 n{in9[3], in9[4], in9[5]};
       vmf_mix mix;
       const bool finite = decode_mix_bf16(reinterpret_cast<const uint16_t *>(outs + threadIdx.x * 10), &mix);
       mix_norms(&mix);
       // a non-finite network output falls back to the reference sampler at this vertex
       const float alpha = finite ? alpha_net : 0.0f;
-      const tpt_material M = S.materials[h.mat];
+      const tpt_material M = S.materials[hmat];
       const bool lights = Sj.n_lights > 0;
       stream R;
       R.init(c.px, c.smp, p.depth + 1, P.jobs[c.job].seed);
-      const v3 atten = tex_value(S, M.tex, u, v, pt);
+      const v3 atten = tex_value<SIMPLE>(S, M.tex, u, v, pt);
       const onb uvw(n);
       v3 nd;
       if (R.next() < alpha) {
         const float u0 = R.next(), u1 = R.next(), u2 = R.next();
         nd = mix_sample(mix, u0, u1, u2);
       } else if (lights && R.next() < 0.5f) {
-        nd = light_random(Sj, pt, R);
+        nd = light_random<SIMPLE>(Sj, pt, R);
       } else {
         nd = uvw.local(on_hemisphere(R));
       }
       const v3 ud = unit(nd);
       // the mixture density first, so the mixture is dead during light_pdf
       const float g = alpha > 0.0f ? mix_pdf(mix, ud) : 0.0f;
-      const float pl = lights ? light_pdf(Sj, pt, nd) : 0.0f;
+      const float pl = lights ? light_pdf<SIMPLE>(Sj, pt, nd) : 0.0f;
       st[ST_LIGHT] += lights ? 1u : 0u;
       float pc = dot(ud, uvw.w);
       pc = pc > 0.0f ? pc : 0.0f;

@@ -717,9 +717,15 @@ void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
 void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
   if (B.guided) {
-    if (B.multi) {
+    if (B.multi && B.simple) {
+      hipLaunchKernelGGL((k_shade<true, true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<true, true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+    } else if (B.multi) {
       hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
       hipLaunchKernelGGL((k_guide_shade<true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+    } else if (B.simple) {
+      hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<false, true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
     } else {
       hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
       hipLaunchKernelGGL((k_guide_shade<false>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
