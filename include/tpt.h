@@ -327,7 +327,7 @@ typedef struct tpt_render_desc {
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
   uint32_t queue_capacity; /* paths in flight; 0 = automatic (2^24, at most the render's samples) */
-  uint32_t flags;          /* TPT_RENDER_PROFILE or 0 */
+  uint32_t flags;          /* TPT_RENDER_* flags or 0 */
   uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
                               the rest with the reference sampler (0 = 2); ignored by tpt_render */
 } tpt_render_desc;
@@ -341,6 +341,12 @@ typedef struct tpt_render_desc {
  * override material::scatter (headers/material.h:74-80), so a path that the
  * medium stops ends there, black. */
 #define TPT_RENDER_MEDIA_SCATTER 2u
+/* flags: drain every pass with trace / shade launches only.  By default, once
+ * a pass has handed out all its work and few paths are left, one fused launch
+ * (k_tail) carries those paths through their remaining bounces (scenes with
+ * constant textures, at most one rect light, no motion or media, on the
+ * small-scene or sphere-only walk); every sample is the same bit for bit. */
+#define TPT_RENDER_NO_TAIL 4u
 
 typedef struct tpt_render_stats {
   uint64_t paths;          /* camera paths completed */
@@ -351,6 +357,9 @@ typedef struct tpt_render_stats {
   uint64_t nodes_visited;  /* pre-order slots visited by closest-hit queries */
   double trace_ms, shade_ms;          /* with TPT_RENDER_PROFILE */
   uint64_t trace_launches, shade_launches;
+  uint64_t tail_segments;  /* segments traced by the fused tail launches (included in segments) */
+  double tail_ms;          /* with TPT_RENDER_PROFILE: time of the fused tail launches */
+  uint64_t tail_launches;
 } tpt_render_stats;
 
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
@@ -387,7 +396,7 @@ typedef struct tpt_launch {
   uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 8 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
                                even); 0 = one launch at a time */
-  uint32_t flags;           /* TPT_RENDER_PROFILE or 0 (profiling launches one at a time) */
+  uint32_t flags;           /* TPT_RENDER_* flags or 0 (profiling launches one at a time) */
   uint32_t guiding;         /* with a guide: training-record stride (0 = none) */
   uint32_t guide_vertices;  /* with a guide: guided diffuse vertices per path (0 = 2) */
 } tpt_launch;

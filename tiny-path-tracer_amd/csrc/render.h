@@ -37,7 +37,7 @@ namespace tptd {
 // Material queues: lambertian hits (the guided ones in guided mode), every other
 // scattering hit, and hits that end the path (emitters, depth limit).
 enum { Q_DIFFUSE = 0, Q_OTHER = 1, Q_TERMINAL = 2 };
-enum { ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_COUNT = 8 };
+enum { ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_TAIL = 5, ST_COUNT = 8 };
 
 // Every queue is split into NSUB regions, each with its own counter 128 bytes
 // from the next: block b appends to region b % NSUB.  Wave atomics on one
@@ -538,6 +538,119 @@ __device__ __forceinline__ void claim_and_append(const render_params &P, const p
   if (cont) store_ray(out, at, p);
 }
 
+// One queued hit (or miss) of class cls with the plain sampler: the
+// reference's color() step for it.  Updates p (radiance, throughput, next
+// ray); true when the path goes on.  Shared by k_shade and k_tail.
+template <bool SIMPLE>
+__device__ __forceinline__ bool shade_hit(const scene_view &S, const scene_view &Sj, const render_params &P,
+                                          const job_dev &J, const item_ctx &c, path &p, const hit_t &h, uint32_t cls,
+                                          uint32_t *st) {
+  bool cont = false;
+  if (h.slot == kNone) {
+    // miss: black (src/utils.cc:85-86) or the sky extension
+    const float bgs = J.background;
+    const v3 bg = bgs > 0.0f ? sky(p.d, bgs) : v3{0.0f, 0.0f, 0.0f};
+    p.rad = p.rad + p.thr * bg;
+  } else {
+    v3 pt, n;
+    float u, v;
+    // (the job's view: a wrapped segment is looked up among its scene's segments only)
+    hit_record(Sj, h, p.o, p.d,
+               !SIMPLE && (P.motion | P.media) ? make_ray_ctx(P, c, p.depth) : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u},
+               &pt, &n, &u, &v);
+    const tpt_material M = S.materials[h.mat];
+    stream R;
+    R.init(c.px, c.smp, p.depth + 1, J.seed);
+    v3 nd{0.0f, 0.0f, 0.0f};
+    if (cls == Q_TERMINAL) {
+      // diffuse_light::emitted is one-sided (src/material.cc:79-86);
+      // every other material emits nothing.
+      v3 e{0.0f, 0.0f, 0.0f};
+      if (M.kind == TPT_MAT_DIFFUSE_LIGHT && dot(n, p.d) < 0.0f) e = tex_value<SIMPLE>(S, M.tex, u, v, pt);
+      p.rad = p.rad + p.thr * e;
+    } else if (M.kind == TPT_MAT_METAL) {
+      // metal::scatter (src/material.cc:88-98); specular: no emitted term.
+      const v3 refl = reflect(unit(p.d), n);
+      nd = refl + M.p[3] * in_unit_sphere(R);
+      if (dot(nd, n) > 0.0f) {
+        p.thr = p.thr * v3{M.p[0], M.p[1], M.p[2]};
+        cont = true;
+      } else {
+        p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
+      }
+    } else if (M.kind == TPT_MAT_ISOTROPIC) {
+      // constant_medium's phase function.  The reference's
+      // isotropic::scatter does not override material::scatter
+      // (headers/material.h:74-80), so color() returns emitted (0): the
+      // path ends black.  TPT_RENDER_MEDIA_SCATTER scatters as the book
+      // intends (src/material.cc:72-77): random_in_unit_sphere, albedo.
+      if (!SIMPLE && P.media_scatter) {
+        nd = in_unit_sphere(R);
+        p.thr = p.thr * tex_value<SIMPLE>(S, M.tex, u, v, pt);
+        cont = true;
+      } else {
+        p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
+      }
+    } else if (M.kind == TPT_MAT_DIELECTRIC) {
+      // dielectric::scatter (src/material.cc:19-70).
+      const float ior = M.p[0];
+      const v3 reflected = reflect(p.d, n);
+      v3 outward;
+      float ni, cosine;
+      if (dot(p.d, n) > 0.0f) {
+        outward = -n;
+        ni = ior;
+        cosine = dot(p.d, n) / len(p.d);
+        cosine = sqrtf(1.0f - ior * ior * (1.0f - cosine * cosine));
+      } else {
+        outward = n;
+        ni = 1.0f / ior;
+        cosine = -dot(p.d, n) / len(p.d);
+      }
+      v3 refracted{0.0f, 0.0f, 0.0f};
+      const float prob = refract(p.d, outward, ni, &refracted) ? schlick(cosine, ior) : 1.0f;
+      nd = R.next() < prob ? reflected : refracted;
+      cont = true;
+    } else {
+      // lambertian + mixture_pdf(hitable_pdf(lights), cosine_pdf)
+      // (src/material.cc:3-17, src/utils.cc:73-81); cosine_pdf::value
+      // is cos θ without 1/π, as in the reference.
+      const v3 atten = tex_value<SIMPLE>(S, M.tex, u, v, pt);
+      const onb uvw(n);
+      float pdf;
+      if (Sj.n_lights > 0) {
+        if (R.next() < 0.5f) {
+          nd = light_random<SIMPLE>(Sj, pt, R);
+        } else {
+          nd = uvw.local(on_hemisphere(R));
+        }
+        const float pl = light_pdf<SIMPLE>(Sj, pt, nd);
+        ++st[ST_LIGHT];
+        float pc = dot(unit(nd), uvw.w);
+        pc = pc > 0.0f ? pc : 0.0f;
+        pdf = 0.5f * pl + 0.5f * pc;
+      } else {
+        nd = uvw.local(on_hemisphere(R));
+        const float pc = dot(unit(nd), uvw.w);
+        pdf = pc > 0.0f ? pc : 0.0f;
+      }
+      float spdf = dot(n, unit(nd));
+      spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+      const v3 w = (atten * spdf) / pdf;
+      p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
+      p.thr = p.thr * w;
+      cont = true;
+      p.ndiff += p.ndiff < 0xFFFFu ? 1u : 0u;  // 16 bits in the record
+    }
+    if (cont) {
+      p.o = pt;
+      p.d = nd;
+      p.depth += 1;
+    }
+  }
+  return cont;
+}
+
 // Shading of the material queues.  Plain mode: all three classes.  Guided
 // mode: the other and terminal classes; the diffuse class (the vertices that
 // use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures, at
@@ -575,110 +688,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
       const float4 hw = Q.mq[cls][3][off];
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
-      const job_dev &J = P.jobs[c.job];
       const scene_view Sj = view_of<MULTI>(S, P, c.job);
-      if (h.slot == kNone) {
-        // miss: black (src/utils.cc:85-86) or the sky extension
-        const float bgs = J.background;
-        const v3 bg = bgs > 0.0f ? sky(p.d, bgs) : v3{0.0f, 0.0f, 0.0f};
-        p.rad = p.rad + p.thr * bg;
-      } else {
-        v3 pt, n;
-        float u, v;
-        // (the job's view: a wrapped segment is looked up among its scene's segments only)
-        hit_record(Sj, h, p.o, p.d,
-                   !SIMPLE && (P.motion | P.media) ? make_ray_ctx(P, c, p.depth) : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u},
-                   &pt, &n, &u, &v);
-        const tpt_material M = S.materials[h.mat];
-        stream R;
-        R.init(c.px, c.smp, p.depth + 1, J.seed);
-        v3 nd{0.0f, 0.0f, 0.0f};
-        if (cls == Q_TERMINAL) {
-          // diffuse_light::emitted is one-sided (src/material.cc:79-86);
-          // every other material emits nothing.
-          v3 e{0.0f, 0.0f, 0.0f};
-          if (M.kind == TPT_MAT_DIFFUSE_LIGHT && dot(n, p.d) < 0.0f) e = tex_value<SIMPLE>(S, M.tex, u, v, pt);
-          p.rad = p.rad + p.thr * e;
-        } else if (M.kind == TPT_MAT_METAL) {
-          // metal::scatter (src/material.cc:88-98); specular: no emitted term.
-          const v3 refl = reflect(unit(p.d), n);
-          nd = refl + M.p[3] * in_unit_sphere(R);
-          if (dot(nd, n) > 0.0f) {
-            p.thr = p.thr * v3{M.p[0], M.p[1], M.p[2]};
-            cont = true;
-          } else {
-            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
-          }
-        } else if (M.kind == TPT_MAT_ISOTROPIC) {
-          // constant_medium's phase function.  The reference's
-          // isotropic::scatter does not override material::scatter
-          // (headers/material.h:74-80), so color() returns emitted (0): the
-          // path ends black.  TPT_RENDER_MEDIA_SCATTER scatters as the book
-          // intends (src/material.cc:72-77): random_in_unit_sphere, albedo.
-          if (!SIMPLE && P.media_scatter) {
-            nd = in_unit_sphere(R);
-            p.thr = p.thr * tex_value<SIMPLE>(S, M.tex, u, v, pt);
-            cont = true;
-          } else {
-            p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};
-          }
-        } else if (M.kind == TPT_MAT_DIELECTRIC) {
-          // dielectric::scatter (src/material.cc:19-70).
-          const float ior = M.p[0];
-          const v3 reflected = reflect(p.d, n);
-          v3 outward;
-          float ni, cosine;
-          if (dot(p.d, n) > 0.0f) {
-            outward = -n;
-            ni = ior;
-            cosine = dot(p.d, n) / len(p.d);
-            cosine = sqrtf(1.0f - ior * ior * (1.0f - cosine * cosine));
-          } else {
-            outward = n;
-            ni = 1.0f / ior;
-            cosine = -dot(p.d, n) / len(p.d);
-          }
-          v3 refracted{0.0f, 0.0f, 0.0f};
-          const float prob = refract(p.d, outward, ni, &refracted) ? schlick(cosine, ior) : 1.0f;
-          nd = R.next() < prob ? reflected : refracted;
-          cont = true;
-        } else {
-          // lambertian + mixture_pdf(hitable_pdf(lights), cosine_pdf)
-          // (src/material.cc:3-17, src/utils.cc:73-81); cosine_pdf::value
-          // is cos θ without 1/π, as in the reference.
-          const v3 atten = tex_value<SIMPLE>(S, M.tex, u, v, pt);
-          const onb uvw(n);
-          float pdf;
-          if (Sj.n_lights > 0) {
-            if (R.next() < 0.5f) {
-              nd = light_random<SIMPLE>(Sj, pt, R);
-            } else {
-              nd = uvw.local(on_hemisphere(R));
-            }
-            const float pl = light_pdf<SIMPLE>(Sj, pt, nd);
-            ++st[ST_LIGHT];
-            float pc = dot(unit(nd), uvw.w);
-            pc = pc > 0.0f ? pc : 0.0f;
-            pdf = 0.5f * pl + 0.5f * pc;
-          } else {
-            nd = uvw.local(on_hemisphere(R));
-            const float pc = dot(unit(nd), uvw.w);
-            pdf = pc > 0.0f ? pc : 0.0f;
-          }
-          float spdf = dot(n, unit(nd));
-          spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
-          const v3 w = (atten * spdf) / pdf;
-          p.rad = p.rad + p.thr * v3{0.0f, 0.0f, 0.0f};  // carries a NaN throughput into the sample
-          p.thr = p.thr * w;
-          cont = true;
-          p.ndiff += p.ndiff < 0xFFFFu ? 1u : 0u;  // 16 bits in the record
-        }
-        if (cont) {
-          p.o = pt;
-          p.d = nd;
-          p.depth += 1;
-        }
-      }
+      cont = shade_hit<SIMPLE>(S, Sj, P, P.jobs[c.job], c, p, h, cls, st);
       if (!cont) {
         finish_sample<GUIDED>(P, Q, p, st);
         finished = true;
@@ -849,6 +860,86 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       }
     }
     claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
+  }
+  block_stats(Q.stats, st, ST_COUNT);
+}
+
+// Fused tail of a pass.  Once every item of the pass is claimed and few paths
+// are left, a bounce costs a trace + shade launch pair whose time is mostly
+// the latency of one wave's work.  k_tail instead takes the live records of
+// ray queue `parity` and carries each path through all its remaining bounces
+// in registers: the trace of k_trace (the same walk), then shade_hit, the
+// shading of k_shade, until the path ends.  Same functions and Philox streams,
+// so every sample equals the bounce loop's bit for bit.  SIMPLE scenes only
+// (constant textures, at most one rect light, no motion or media).
+//
+// Guided mode: a path that reaches a guided diffuse vertex needs the network
+// (k_guide_shade, a wave-wide MFMA pass); it is handed back instead, its
+// record and hit appended to diffuse material queue `parity` exactly as
+// k_trace appends them, for the caller's next shade launch.  Ray queue
+// parity ^ 1 (where that launch appends) is cleared here, as k_trace does.
+template <bool SWEEP, bool MULTI, bool SPH, bool GUIDED>
+__global__ void __launch_bounds__(256) k_tail(scene_view S, const tpt_slot *__restrict__ slots,
+                                              const tpt_entry *__restrict__ entries, render_params P,
+                                              render_state Q, uint32_t parity) {
+  static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");
+  static_assert(!(SWEEP && SPH), "the sweep takes every leaf kind");
+  S.slots = slots;
+  S.entries = entries;
+  __shared__ uint32_t offs[NSUB];
+  __shared__ uint32_t pre[NSUB + 1];
+  if (threadIdx.x < NSUB) offs[threadIdx.x] = rc_at(parity, threadIdx.x);
+  __syncthreads();
+  region_prefix(Q.counters, offs, NSUB, pre);
+  const uint32_t count = pre[NSUB];
+  if (blockIdx.x == 0 && threadIdx.x < NSUB) Q.counters[rc_at(parity ^ 1u, threadIdx.x)] = 0;
+  float4 *const *in = Q.rq[parity];
+  const uint32_t home = blockIdx.x % NSUB;
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    const uint32_t qi = base + threadIdx.x;
+    bool live = qi < count;
+    path p{};
+    if (live) {
+      const uint32_t r = find_region(pre, NSUB, qi);
+      p = load_ray(in, r * Q.region + (qi - pre[r]));
+    }
+    // wave-uniform: the sweep needs every lane of the wave
+    while (__ballot(live) != 0ull) {
+      const item_ctx c = live ? item_in_entry(P, MULTI ? p.ent : 0u, p.item) : item_ctx{0u, 0u, 0u, 0u};
+      const scene_view Sj = view_of<MULTI>(S, P, c.job);
+      hit_t h{0.0f, kNone, kNone, 0};
+      const bool got = trace_world<SWEEP, SPH, false>(Sj, p.o, p.d, ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u}, 0.001f,
+                                                      kFltMax, &h, &st[ST_NODES], live);
+      uint32_t cls = Q_TERMINAL;
+      if (live) {
+        ++st[ST_SEGMENTS];
+        ++st[ST_TAIL];
+        if (got) {
+          h.mat &= TPT_PAYLOAD_MASK;
+          const uint32_t kind = S.materials[h.mat].kind;
+          if (kind != TPT_MAT_DIFFUSE_LIGHT && p.depth < P.jobs[c.job].max_depth)
+            cls = kind == TPT_MAT_LAMBERTIAN && p.ndiff < P.guide_verts ? Q_DIFFUSE : Q_OTHER;
+        } else {
+          h.slot = kNone;
+        }
+      }
+      if (GUIDED) {
+        const bool back = live && cls == Q_DIFFUSE;
+        const uint32_t at = home * Q.region + wave_append(Q.counters + mc_at(parity, Q_DIFFUSE, home), back);
+        if (back) {
+          store_ray(Q.mq[Q_DIFFUSE], at, p);
+          Q.mq[Q_DIFFUSE][3][at] =
+              make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
+          live = false;
+        }
+      }
+      if (live && !shade_hit<true>(S, Sj, P, P.jobs[c.job], c, p, h, cls, st)) {
+        finish_sample<GUIDED>(P, Q, p, st);
+        live = false;
+      }
+    }
   }
   block_stats(Q.stats, st, ST_COUNT);
 }

@@ -692,7 +692,35 @@ struct bounce_launch {
   bool sweep, multi, guided;
   bool sph;     // every leaf of the launch's scenes is a sphere
   bool simple;  // their textures are constant and their light root absent or one xz_rect
+  bool tail;    // passes end with a fused k_tail launch (simple scenes, sweep or sphere-only walk)
 };
+
+// The fused tail on ray queue `parity` (k_tail), `blocks` blocks.
+void launch_tail(const bounce_launch &B, hipStream_t st, uint32_t parity, uint32_t blocks) {
+  const dim3 g(blocks), b(256);
+  if (B.sweep) {
+    if (B.guided)
+      hipLaunchKernelGGL((k_tail<true, false, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+    else
+      hipLaunchKernelGGL((k_tail<true, false, false, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+  } else if (B.multi) {
+    if (B.guided)
+      hipLaunchKernelGGL((k_tail<false, true, true, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+    else
+      hipLaunchKernelGGL((k_tail<false, true, true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+  } else {
+    if (B.guided)
+      hipLaunchKernelGGL((k_tail<false, false, true, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+    else
+      hipLaunchKernelGGL((k_tail<false, false, true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
+                         parity);
+  }
+}
 
 void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
@@ -745,7 +773,7 @@ void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
 int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch &L, tpt_guide *guide,
                 tpt_render_stats *stats, void *stream_ptr) {
   if (!s || (n_jobs && !jobs)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
-  if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER))
+  if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL))
     return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
   if (L.queue_capacity > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
@@ -848,6 +876,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   }
   B.multi = multi;
   B.guided = guide != nullptr;
+  B.tail = !(L.flags & TPT_RENDER_NO_TAIL) && B.simple && (B.sweep || B.sph);
   render_params &P = B.P;
   P.jobs = s->d_jobs;
   P.scenes = s->d_scenes;
@@ -936,6 +965,30 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     for (uint32_t k = 0; k < NSUB; ++k) *live += s->host_count[k * CSTRIDE];
     return TPT_OK;
   };
+  // The fused tail (k_tail) takes over a pass once it has handed out every
+  // item (fewer live records than the capacity) and at most tail_live paths
+  // are left: from there on a bounce launch pair costs mostly latency.  Polls
+  // come every `poll` iterations, the first after kPollFirst, and every
+  // kPollTail once the items are out, so the switch comes early.
+  constexpr uint64_t kPollFirst = 8, kPollTail = 4;
+  const uint64_t tail_live = std::min<uint64_t>(cap / 4, 1u << 18);
+  double tail_ms = 0.0;
+  uint64_t n_tail = 0;
+  auto run_tail = [&](uint32_t q, uint64_t live) -> int {
+    // at least NSUB blocks (the hand-back appends stay within their regions)
+    const uint32_t blocks =
+        static_cast<uint32_t>(std::max<uint64_t>(NSUB, std::min<uint64_t>(grid, (live + 255) / 256)));
+    if (profile) HIP_TRY(hipEventRecord(s->events[0], ws));
+    launch_tail(B, ws, q, blocks);
+    if (profile) {
+      HIP_TRY(hipEventRecord(s->events[1], ws));
+      HIP_TRY(hipEventSynchronize(s->events[1]));
+      float a = 0.0f;
+      if (hipEventElapsedTime(&a, s->events[0], s->events[1]) == hipSuccess) tail_ms += a;
+    }
+    ++n_tail;
+    return TPT_OK;
+  };
   bool drained = true;
   for (const pass_plan &pp : plan) {
     if (!drained) break;
@@ -957,6 +1010,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     // least one path per live record, so the loop is bounded.
     const uint64_t max_iter = (static_cast<uint64_t>(pp.n_items) / cap + 2) * (max_depth + 2ull) + 64;
     drained = false;
+    bool exhausted = pp.n_items <= cap;
     if (use_graph) {
       for (uint64_t it = 0; it < max_iter; it += K) {
         HIP_TRY(hipGraphLaunch(s->gexec, ws));
@@ -966,9 +1020,28 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
           drained = true;
           break;
         }
+        exhausted = exhausted || live < cap;
+        if (B.tail && exhausted && live <= tail_live) {
+          if ((rc = run_tail(0, live)) != TPT_OK) return rc;
+          if (!B.guided) {
+            drained = true;
+            break;
+          }
+          // handed-back guided vertices: shade them and bring their paths
+          // back to queue 0, where the next replay starts
+          launch_shade(B, ws, 0);
+          launch_trace(B, ws, 1);
+          launch_shade(B, ws, 1);
+          if ((rc = live_count(0, &live)) != TPT_OK) return rc;
+          if (live == 0) {
+            drained = true;
+            break;
+          }
+        }
       }
     } else {
       int batch = 0;
+      uint64_t next_poll = (B.tail ? kPollFirst : static_cast<uint64_t>(poll)) - 1;
       for (uint64_t it = 0; it < max_iter; ++it) {
         const uint32_t parity = static_cast<uint32_t>(it & 1u);
         if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch], ws));
@@ -977,7 +1050,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
         launch_shade(B, ws, parity);
         if (profile) HIP_TRY(hipEventRecord(s->events[2 * batch + 2], ws));
         ++batch;
-        if ((it % poll) == poll - 1) {
+        if (it == next_poll) {
           uint64_t live = 0;
           if ((rc = live_count(parity ^ 1u, &live)) != TPT_OK) return rc;
           if (profile) harvest(batch);
@@ -986,6 +1059,24 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
             drained = true;
             break;
           }
+          exhausted = exhausted || live < cap;
+          if (B.tail && exhausted && live <= tail_live) {
+            if ((rc = run_tail(parity ^ 1u, live)) != TPT_OK) return rc;
+            if (!B.guided) {
+              drained = true;
+              break;
+            }
+            // handed-back guided vertices: shade them; their paths continue
+            // in queue `parity`, the input of iteration it + 2
+            launch_shade(B, ws, parity ^ 1u);
+            ++it;
+            if ((rc = live_count(parity, &live)) != TPT_OK) return rc;
+            if (live == 0) {
+              drained = true;
+              break;
+            }
+          }
+          next_poll = it + (B.tail && exhausted ? kPollTail : static_cast<uint64_t>(poll));
         }
       }
       if (profile && batch > 0) {
@@ -1028,6 +1119,9 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     stats->shade_ms = shade_ms;
     stats->trace_launches = n_launch;
     stats->shade_launches = n_launch;
+    stats->tail_segments = tot[ST_TAIL];
+    stats->tail_ms = tail_ms;
+    stats->tail_launches = n_tail;
   }
   return TPT_OK;
 }
