@@ -364,8 +364,10 @@ __global__ void k_zero_stats(unsigned long long *stats, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) stats[i] = 0;
 }
 
-// First items of a pass, straight into ray queue 0 (item w -> region w % NSUB,
-// entry w / NSUB; no atomics); resets every counter of the pass.
+// First items of a pass, straight into ray queue 0 (no atomics); resets every
+// counter of the pass.  Items go in runs of 64, one wave's stores: run
+// m = w / 64 fills positions 64 (m / NSUB) .. + 63 of region m % NSUB, so
+// each store instruction writes 1 KB of consecutive addresses.
 template <bool MULTI>
 __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
   const pass_hdr H = *P.hdr;
@@ -374,12 +376,19 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
   for (uint32_t w = gid; w < COUNTER_WORDS / CSTRIDE; w += stride) {
     const uint32_t set = w / NSUB, k = w % NSUB;
     uint32_t v = 0;
-    if (set == 0) v = (H.n_first + NSUB - 1 - k) / NSUB;  // ray queue 0 region counts
-    if (set == 8) v = shard_lo(H, k);                       // shard cursors
+    if (set == 0) {
+      // ray queue 0 region counts: the full runs of region k, and the last,
+      // partial run if it falls there
+      const uint32_t full = H.n_first / 64, part = H.n_first % 64;
+      v = 64 * (full / NSUB + (k < full % NSUB ? 1u : 0u)) + (part && full % NSUB == k ? part : 0u);
+    }
+    if (set == 8) v = shard_lo(H, k);  // shard cursors
     Q.counters[w * CSTRIDE] = v;
   }
-  for (uint32_t w = gid; w < H.n_first; w += stride)
-    store_ray(Q.rq[0], (w % NSUB) * Q.region + w / NSUB, raygen_item<MULTI>(P, H, w));
+  for (uint32_t w = gid; w < H.n_first; w += stride) {
+    const uint32_t m = w / 64;
+    store_ray(Q.rq[0], (m % NSUB) * Q.region + 64 * (m / NSUB) + w % 64, raygen_item<MULTI>(P, H, w));
+  }
 }
 
 // SWEEP: wave-uniform traversal for small scenes (see segment_sweep).  SPH:
