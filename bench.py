@@ -359,7 +359,8 @@ def main():
         sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf)
     torch.cuda.synchronize()
     totals = {"segments": 0, "paths": 0, "trace_ms": 0.0, "shade_ms": 0.0, "trace_launches": 0,
-              "shade_launches": 0, "light_queries": 0, "nodes_visited": 0, "nan_samples": 0}
+              "shade_launches": 0, "light_queries": 0, "nodes_visited": 0, "nan_samples": 0, "tail_segments": 0,
+              "tail_ms": 0.0, "tail_launches": 0}
     t0 = time.perf_counter()
     for k in range(args.steps):
         sum_buf.zero_()
@@ -373,8 +374,11 @@ def main():
     value = samples / dt / 1e6
     segs = totals["segments"]
     paths = totals["paths"]
-    trace_bytes = TRACE_BYTES_PER_SEGMENT * segs
-    shade_bytes = SHADE_BYTES_PER_SEGMENT * segs + ACCUM_BYTES_PER_PATH * paths
+    # the bounce-loop kernels' share: the last paths of each pass are traced and
+    # shaded by the fused tail launch (k_tail), timed apart
+    loop_segs = segs - totals["tail_segments"]
+    trace_bytes = TRACE_BYTES_PER_SEGMENT * loop_segs
+    shade_bytes = SHADE_BYTES_PER_SEGMENT * loop_segs + ACCUM_BYTES_PER_PATH * paths
     if totals["trace_ms"] >= totals["shade_ms"]:
         kname, kbytes, kms, nl = "k_trace", trace_bytes, totals["trace_ms"], totals["trace_launches"]
     else:
@@ -388,7 +392,7 @@ def main():
         with open(tfile) as f:
             tk = json.load(f)["kernels"].get(kname)
         if tk:
-            traffic = round(tk["hbm_bytes_per_segment"] * segs / max(nl, 1))
+            traffic = round(tk["hbm_bytes_per_segment"] * loop_segs / max(nl, 1))
     out = {
         "metric": "Msamples/s (plain GPU tracer, Cornell 1200^2, 2048 spp, depth 50)",
         "value": round(value, 3),
@@ -412,10 +416,14 @@ def main():
                    "light_queries": totals["light_queries"], "nodes_per_segment":
                        round(totals["nodes_visited"] / max(segs, 1), 2), "trace_ms": round(totals["trace_ms"], 1),
                    "shade_ms": round(totals["shade_ms"], 1), "nan_samples": totals["nan_samples"],
+                   "tail_segments": totals["tail_segments"], "tail_ms": round(totals["tail_ms"], 1),
+                   "tail_launches": totals["tail_launches"],
                    # BVH / primitive bytes the walk requests: one 32-byte slot per visit (boxes and
                    # leaves alike), served from L1/L2 (the Cornell box's slots are 576 bytes)
                    "bvh_slot_bytes_per_segment": round(32 * totals["nodes_visited"] / max(segs, 1), 1),
-                   "bvh_requested_GBps": round(32 * totals["nodes_visited"] / (totals["trace_ms"] / 1e3) / 1e9, 1)
+                   # (over the trace launches and the fused tail launches, which also walk)
+                   "bvh_requested_GBps": round(32 * totals["nodes_visited"] /
+                                               ((totals["trace_ms"] + totals["tail_ms"]) / 1e3) / 1e9, 1)
                    if totals["trace_ms"] else None},
     }
     if not args.no_cpu:
