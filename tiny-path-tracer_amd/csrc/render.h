@@ -805,7 +805,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         }
       }
     }
-    __syncthreads();
+    // Each wave reads only the table rows its own MFMA passes wrote, so a
+    // wave-level barrier suffices (a wave's LDS operations complete in
+    // order) and the block's waves need not wait for each other.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // Phase 2: guided shading of this lane's entry.
     bool cont = false, finished = false;
     path p{};
