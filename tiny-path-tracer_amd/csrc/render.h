@@ -725,14 +725,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
 //
 // SIMPLE: as for k_shade (constant textures, at most one rect light, no
 // moving spheres).
+// Block size of k_guide_shade: its LDS holds the inference tiles (30 KB) once
+// per block plus 80 B of outputs per thread.  SIMPLE fits 128 VGPRs, so
+// 512-thread blocks (73 KB, two per CU) give 4 waves per SIMD; the general
+// variant needs more registers and runs 3 blocks of 256 per CU.
+__host__ __device__ constexpr uint32_t guide_shade_threads(bool simple) { return simple ? 512u : 256u; }
+__host__ __device__ constexpr uint32_t guide_shade_blocks_per_cu(bool simple) { return simple ? 2u : 3u; }
+
 template <bool MULTI, bool SIMPLE = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+__global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((amdgpu_waves_per_eu(SIMPLE ? 4 : 3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
   __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
   __shared__ float bias[G_BIAS];
   __shared__ uint32_t offs[NSUB];
   __shared__ uint32_t pre[NSUB + 1];
-  __shared__ uint2 outs[256 * 10];  // entry e: outputs 4k..4k+3 as bf16 in outs[10 e + k]
+  __shared__ uint2 outs[guide_shade_threads(SIMPLE) * 10];  // entry e: outputs 4k..4k+3 as bf16 in outs[10 e + k]
   guide_stage_weights(wt, P.g_tiles, GX_TILES * GX_TILE_WORDS);
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = P.g_params[G_B1 + i];
   if (threadIdx.x < NSUB) offs[threadIdx.x] = mc_at(parity, Q_DIFFUSE, threadIdx.x);

@@ -199,14 +199,15 @@ void free_scene(tpt_scene *s) {
 }
 
 // Region size: block b of a grid of G appends to region b % NSUB.  Live
-// records never exceed `capacity`; each block takes grid-stride chunks of 256,
-// so it handles at most t / G + 256 of t entries and the ceil(G / NSUB) blocks
-// of a residue at most t / NSUB + t / G + ceil(G / NSUB) * 256.  Two kernels
-// (grids G1, G2) may append to the same ray queue, their t adding up to at most
-// `capacity`.
+// records never exceed `capacity`; each block takes grid-stride chunks of its
+// block size B, so it handles at most t / G + B of t entries and the
+// ceil(G / NSUB) blocks of a residue at most t / NSUB + t / G + ceil(G / NSUB) * B.
+// Two kernels (grid G1 of 256-thread blocks, grid G2 of k_guide_shade's
+// blocks, at most 512 threads) may append to the same ray queue, their t adding
+// up to at most `capacity`.
 size_t region_size(uint32_t capacity, uint32_t g1, uint32_t g2) {
   const size_t r1 = (g1 + NSUB - 1) / NSUB, r2 = (g2 + NSUB - 1) / NSUB;
-  return capacity / NSUB + capacity / g1 + capacity / g2 + (r1 + r2) * 256 + 256;
+  return capacity / NSUB + capacity / g1 + capacity / g2 + r1 * 256 + r2 * 512 + 256;
 }
 
 int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t grid, uint32_t guide_grid) {
@@ -745,18 +746,19 @@ void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
 void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
   if (B.guided) {
+    const dim3 gg(B.guide_grid), gbs(guide_shade_threads(true)), gbg(guide_shade_threads(false));
     if (B.multi && B.simple) {
       hipLaunchKernelGGL((k_shade<true, true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
-      hipLaunchKernelGGL((k_guide_shade<true, true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<true, true>), gg, gbs, 0, st, B.S, B.P, B.Q, parity);
     } else if (B.multi) {
       hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
-      hipLaunchKernelGGL((k_guide_shade<true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<true>), gg, gbg, 0, st, B.S, B.P, B.Q, parity);
     } else if (B.simple) {
       hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
-      hipLaunchKernelGGL((k_guide_shade<false, true>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<false, true>), gg, gbs, 0, st, B.S, B.P, B.Q, parity);
     } else {
       hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, B.S, B.P, B.Q, parity);
-      hipLaunchKernelGGL((k_guide_shade<false>), dim3(B.guide_grid), b, 0, st, B.S, B.P, B.Q, parity);
+      hipLaunchKernelGGL((k_guide_shade<false>), gg, gbg, 0, st, B.S, B.P, B.Q, parity);
     }
   } else if (B.multi && B.simple) {
     hipLaunchKernelGGL((k_shade<false, true, true>), g, b, 0, st, B.S, B.P, B.Q, parity);
@@ -851,8 +853,11 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
 
   const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 8;
   const uint32_t grid = static_cast<uint32_t>(s->n_cu) * bpc;
-  // k_guide_shade holds 51 KB of LDS (weights + output table): 3 blocks per CU
-  const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * 3;
+  // k_guide_shade's LDS (weights + output table) allows 2 blocks of 512 per CU
+  // (simple scenes) or 3 of 256
+  bool all_simple = true;
+  for (uint32_t j = 0; j < n_jobs; ++j) all_simple = all_simple && s->simple[jobs[j].scene];
+  const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * guide_shade_blocks_per_cu(all_simple);
   // 2^24 records in flight by default (4.8 GB of queues): launches long enough
   // that their ramp-up and tail cost little
   uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 24);
