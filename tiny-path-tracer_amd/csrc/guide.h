@@ -262,9 +262,11 @@ __device__ __forceinline__ bool decode_mix_bf16(const uint16_t *o, vmf_mix *m) {
   return finite;
 }
 
-// vMF density exp(κ(μ·ω − 1)) · κ / (2π (1 − e^{−2κ})) on the unit sphere.
+// vMF density exp(κ(μ·ω − 1)) · κ / (2π (1 − e^{−2κ})) on the unit sphere
+// (κ ≥ e^{−3}, the log-κ clamp: 1 − e^{−2κ} ≥ 0.095 loses nothing to the
+// plain difference).
 __device__ __forceinline__ float vmf_pdf(float kappa, float cosang) {
-  const float norm = kappa / (2.0f * kPiF * (-expm1f(-2.0f * kappa)));
+  const float norm = __fdividef(kappa, 2.0f * kPiF * (1.0f - __expf(-2.0f * kappa)));
   return norm * __expf(kappa * (cosang - 1.0f));
 }
 

@@ -186,13 +186,15 @@ __device__ __forceinline__ void lobe_eval(float a, float b, float c, float lk, v
 __device__ __forceinline__ void lobe_grad(const lobe &L, v3 om, float f, float g, float *o5) {
   const float wd = L.w * L.d;
   // d/d raw mean: κ (ω − (μ·ω) μ) / |raw|
-  const float s = L.nrm > 1e-10f ? f * wd * L.kap / L.nrm : 0.0f;
+  const float s = L.nrm > 1e-10f ? __fdividef(f * wd * L.kap, L.nrm) : 0.0f;
   o5[0] = s * (om.x - L.cs * L.mu[0]);
   o5[1] = s * (om.y - L.cs * L.mu[1]);
   o5[2] = s * (om.z - L.cs * L.mu[2]);
   // d/d log κ: 1 − 2κ/(e^{2κ} − 1) + κ(μ·ω − 1), zero where the clamp is active
+  // (2κ ≥ 0.0996: e^{2κ} − 1 ≥ 0.105 keeps ~6 digits; e^{2κ} = inf gives 0)
   const bool live = L.lk > G_LOGK_MIN && L.lk < G_LOGK_MAX;
-  o5[3] = live ? f * wd * (1.0f - 2.0f * L.kap / expm1f(2.0f * L.kap) + L.kap * (L.cs - 1.0f)) : 0.0f;
+  o5[3] = live ? f * wd * (1.0f - __fdividef(2.0f * L.kap, __expf(2.0f * L.kap) - 1.0f) + L.kap * (L.cs - 1.0f))
+               : 0.0f;
   // d/d logit: w_i (D_i − g)
   o5[4] = f * L.w * (L.d - g);
 }
@@ -299,13 +301,13 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
     float z = ea + eb;
     z += __shfl_xor(z, 16);
     z += __shfl_xor(z, 32);
-    const float iz = 1.0f / z;
+    const float iz = __fdividef(1.0f, z);
     A.w = ea * iz, B.w = eb * iz;
     float gp = A.w * A.d + B.w * B.d;
     gp += __shfl_xor(gp, 16);
     gp += __shfl_xor(gp, 32);
     gp = fmaxf(gp, 1e-30f);
-    const float f = use ? -c * inv_n / gp : 0.0f;
+    const float f = use ? __fdividef(-c * inv_n, gp) : 0.0f;
     float ga[5], gb[5];
     lobe_grad(A, om, f, gp, ga);
     lobe_grad(B, om, f, gp, gb);
@@ -316,7 +318,7 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
     if (use && g == 0) {
       loss += -c * __logf(gp);
       const float mm = alpha * gp + (1.0f - alpha) * qd;
-      if (mm > 0.0f) dalpha += -c * inv_n * (gp - qd) * alpha * (1.0f - alpha) / mm;
+      if (mm > 0.0f) dalpha += __fdividef(-c * inv_n * (gp - qd) * alpha * (1.0f - alpha), mm);
     }
     // ---- 3..5, layer by layer from the top, keeping few operands live:
     // neuron-major operands (lane: neuron 16 t + j; registers: samples 4g + r)
