@@ -44,9 +44,10 @@ def parse():
     p.add_argument("--scene", default="cornell555")
     p.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     p.add_argument("--ref-spp", type=int, default=65536, help="spp of the converged plain render RMSE is taken against")
-    p.add_argument("--workload", default="cornell", choices=["cornell", "batch64", "guide_train"],
+    p.add_argument("--workload", default="cornell", choices=["cornell", "batch64", "guide_train", "guided"],
                    help="cornell: the headline frame; batch64: 64 uneven random-sphere renders in one batch; "
-                        "guide_train: Adam steps of the guiding network on 2^18 rows")
+                        "guide_train: Adam steps of the guiding network on 2^18 rows; guided: the guided "
+                        "estimator's frames after the online training passes")
     p.add_argument("--launch", default=None, metavar="CAP,BLOCKS,GRAPH",
                    help="batch64: use this launch (queue capacity, blocks per CU, graph iterations) instead of "
                         "autotuning")
@@ -325,11 +326,58 @@ def guide_train(tpt, torch, args):
     }
 
 
+def guided(tpt, torch, args):
+    """One step = one frame of `spp` (default 256) samples per pixel with the
+    guided estimator, after the online schedule's training passes (not timed:
+    the time-to-RMSE figure of the cornell workload includes them)."""
+    sc = tpt.Scene.named(args.scene)
+    info = sc.host.info
+    W, H = info.width, info.height
+    spp = args.spp or 256
+    sch = GUIDED_SCHEDULE
+    g = tpt.Guide(seed=1)
+    rg = tpt.ProgressiveRender(sc, guide=g, seed=1, record_stride=sch["record_stride"], iters=sch["iters"],
+                               batch=sch["batch"], lr=sch["lr"])
+    t0 = time.perf_counter()
+    for k in range(sch["train_passes"]):
+        rg.add(sch["pass_spp"], train=True)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    sum_buf = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    sq_buf = torch.zeros_like(sum_buf)
+    for i in range(args.warmup):
+        sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g)
+    torch.cuda.synchronize()
+    segs = paths = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sum_buf.zero_()
+        sq_buf.zero_()
+        _, _, st = sc.render(spp=spp, seed=k + 1, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g)
+        segs += st["segments"]
+        paths += st["paths"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = args.steps * W * H * spp / dt / 1e6
+    return {
+        "metric": f"Msamples/s (guided GPU tracer, {args.scene} {W}x{H}, {spp} spp, depth {info.max_depth})",
+        "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (network bf16)",
+        "data": "synthetic (procedural scene built through the scene API; the guide trained online on its own rows)",
+        "config": {"workload": f"{args.scene} {W}x{H}, {spp} spp, guided estimator (2 guided vertices)",
+                   "schedule": sch},
+        "detail": {"Mrays_per_s": round(segs / dt / 1e6, 2), "segments_per_path": round(segs / max(paths, 1), 3),
+                   "training_s": round(t_train, 4),
+                   "guided_mix_weight": round(float(1 / (1 + np.exp(-g.get_params()[-1]))), 3)},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported (the path would shard by "
-              "pixel tiles / sample ranges with one accumulator reduce, see DESIGN.md)", file=sys.stderr)
+              "pixel tiles / sample ranges with one accumulator reduce)", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -339,6 +387,9 @@ def main():
         return
     if args.workload == "guide_train":
         print(json.dumps(guide_train(tpt, torch, args)), flush=True)
+        return
+    if args.workload == "guided":
+        print(json.dumps(guided(tpt, torch, args)), flush=True)
         return
     if args.copy_calibration:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
