@@ -148,47 +148,50 @@ __device__ __forceinline__ void guide_xfeatures(int g, float p, const float *nwo
   xb[1] = pack8(f32x4{v[8], v[9], v[10], v[11]}, f32x4{v[12], v[13], v[14], v[15]});
 }
 
+// ReLU of 8 packed bf16 values: as 16-bit integers a negative bf16 (sign bit
+// set, -0 included) is below 0 and a positive one orders like its value, so an
+// integer max with 0 is the ReLU (v_pk_max_i16, two values per instruction).
+// Rounding to bf16 keeps signs and order, so ReLU after the conversion gives
+// the same bits as before it.
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 relu_bf16x8(bf16x8 v) {
+  const s16x8 x = __builtin_bit_cast(s16x8, v);
+  return __builtin_bit_cast(bf16x8, __builtin_elementwise_max(x, s16x8{0, 0, 0, 0, 0, 0, 0, 0}));
+}
+
 // Forward pass of 16 samples (one per lane column l%16) in the 16x16x32
 // layout: xb = this lane's first-layer B operands, xt = LDS inference tiles,
-// bias = LDS biases.  out[mt][r] = output 16 mt + 4 (l/16) + r.
+// bias = LDS biases.  out[mt][r] = output 16 mt + 4 (l/16) + r.  The
+// accumulators start at the biases (no separate bias add).
 __device__ __forceinline__ void guide_forward_x32(const bf16x8 xb_in[2], const uint32_t *xt, const float *bias,
                                                   f32x4 out[3]) {
   const int l = __lane_id();
   const int fg = l >> 4;
   const bf16x8 *tiles = reinterpret_cast<const bf16x8 *>(xt);
+  const f32x4 *bias4 = reinterpret_cast<const f32x4 *>(bias);  // bias4[4 t + fg]: rows 16 t + 4 fg .. + 3
   bf16x8 xb[2] = {xb_in[0], xb_in[1]};
 #pragma unroll
   for (int layer = 0; layer < 3; ++layer) {
     f32x4 acc[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      acc[mt] = bias4[4 * (4 * layer + mt) + fg];
 #pragma unroll
       for (int s = 0; s < 2; ++s)
         acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tiles[(GX_BASE[layer] + 2 * mt + s) * 64 + l], xb[s],
                                                           acc[mt], 0, 0, 0);
     }
-    const float *b = bias + 64 * layer;
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float z = acc[mt][r] + b[16 * mt + 4 * fg + r];
-        acc[mt][r] = z > 0.0f ? z : 0.0f;
-      }
-    xb[0] = pack8(acc[0], acc[1]);
-    xb[1] = pack8(acc[2], acc[3]);
+    xb[0] = relu_bf16x8(pack8(acc[0], acc[1]));
+    xb[1] = relu_bf16x8(pack8(acc[2], acc[3]));
     __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int mt = 0; mt < 3; ++mt) {
-    out[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    out[mt] = bias4[4 * (12 + mt) + fg];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
       out[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tiles[(GX_BASE[3] + 2 * mt + s) * 64 + l], xb[s], out[mt], 0,
                                                         0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) out[mt][r] += bias[192 + 16 * mt + 4 * fg + r];
   }
 }
 

@@ -130,7 +130,7 @@ __device__ __forceinline__ void guide_stage_weights(uint32_t *dst, const uint32_
 __global__ void __launch_bounds__(256) k_guide_infer(const uint32_t *__restrict__ xtiles, const float *__restrict__ params,
                                                      const float *__restrict__ in, uint64_t n, float *__restrict__ out) {
   __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
-  __shared__ float bias[G_BIAS];
+  __shared__ __align__(16) float bias[G_BIAS];
   guide_stage_weights(wt, xtiles, GX_TILES * GX_TILE_WORDS);
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) bias[i] = params[G_B1 + i];
   __syncthreads();

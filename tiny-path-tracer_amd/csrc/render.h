@@ -736,7 +736,7 @@ template <bool MULTI, bool SIMPLE = false>
 __global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((amdgpu_waves_per_eu(SIMPLE ? 4 : 3))) k_guide_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
   __shared__ uint32_t wt[GX_TILES * GX_TILE_WORDS];
-  __shared__ float bias[G_BIAS];
+  __shared__ __align__(16) float bias[G_BIAS];
   __shared__ uint32_t offs[NSUB];
   __shared__ uint32_t pre[NSUB + 1];
   __shared__ uint2 outs[guide_shade_threads(SIMPLE) * 10];  // entry e: outputs 4k..4k+3 as bf16 in outs[10 e + k]
