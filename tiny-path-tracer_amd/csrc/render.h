@@ -664,10 +664,11 @@ __device__ __forceinline__ bool shade_hit(const scene_view &S, const scene_view 
 // mode: the other and terminal classes; the diffuse class (the vertices that
 // use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures, at
 // most one rect light (see shade.h) and no moving spheres.  The occupancy targets below make the
-// register allocator fit 80 or 96 registers (6 or 5 waves per SIMD instead of
-// 4) without spilling.
+// register allocator fit 96 registers (5 waves per SIMD instead of 4) without
+// spilling, or, for the single-scene SIMPLE variant, 64 (8 waves per SIMD,
+// with ten registers spilled: measured 3% faster than 6 waves without spills).
 template <bool GUIDED, bool MULTI, bool SIMPLE = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE ? (MULTI ? 5 : 6) : (GUIDED ? 1 : 5)))) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE ? (MULTI ? 5 : 8) : (GUIDED ? 1 : 5)))) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
   // material regions in class-major order: region r = (class - C0) * NSUB + k
   constexpr uint32_t C0 = GUIDED ? 1u : 0u, NR = (3u - C0) * NSUB;
