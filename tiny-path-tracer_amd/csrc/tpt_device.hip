@@ -851,7 +851,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   }
   HIP_TRY(hipMemcpyAsync(s->d_jobs, jd.data(), n_jobs * sizeof(job_dev), hipMemcpyHostToDevice, ws));
 
-  const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 8;
+  const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 16;
   const uint32_t grid = static_cast<uint32_t>(s->n_cu) * bpc;
   // k_guide_shade's LDS (weights + output table) allows 2 blocks of 512 per CU
   // (simple scenes) or 3 of 256
