@@ -393,7 +393,7 @@ typedef struct tpt_job {
 /* Launch settings of one tpt_render_jobs call. */
 typedef struct tpt_launch {
   uint32_t queue_capacity;  /* path records in flight; 0 = 2^24 */
-  uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 16 */
+  uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 32 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
                                even); 0 = one launch at a time */
   uint32_t flags;           /* TPT_RENDER_* flags or 0 (profiling launches one at a time) */
@@ -409,7 +409,7 @@ int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, cons
 
 /* Autotuner: times calibration renders of the jobs (at most calib_spp
  * samples each, 0 = 2, into scratch sums) over queue capacities (2^21..2^24),
- * grid sizes (4, 8, 16 blocks per CU) and graph replay lengths (0, 8), and
+ * grid sizes (8, 16, 32 blocks per CU) and graph replay lengths (0, 8), and
  * writes the fastest settings into *best (its flags, guiding and
  * guide_vertices are kept).  *best_ms: that calibration's wall time. */
 int tpt_autotune(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *guide, uint32_t calib_spp,

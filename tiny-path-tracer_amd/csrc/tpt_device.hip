@@ -851,7 +851,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   }
   HIP_TRY(hipMemcpyAsync(s->d_jobs, jd.data(), n_jobs * sizeof(job_dev), hipMemcpyHostToDevice, ws));
 
-  const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 16;
+  const uint32_t bpc = L.blocks_per_cu ? L.blocks_per_cu : 32;
   const uint32_t grid = static_cast<uint32_t>(s->n_cu) * bpc;
   // k_guide_shade's LDS (weights + output table) allows 2 blocks of 512 per CU
   // (simple scenes) or 3 of 256
@@ -1199,7 +1199,7 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
     cj[j].d_sumsq = nullptr;
   }
   const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
-  const uint32_t bpcs[] = {4, 8, 16};
+  const uint32_t bpcs[] = {8, 16, 32};
   const uint32_t graphs[] = {0, 8};
   tpt_launch base = *best;
   base.flags = 0;
