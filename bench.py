@@ -445,7 +445,8 @@ def main():
         if tk:
             traffic = round(tk["hbm_bytes_per_segment"] * loop_segs / max(nl, 1))
     out = {
-        "metric": "Msamples/s (plain GPU tracer, Cornell 1200^2, 2048 spp, depth 50)",
+        "metric": ("Msamples/s (plain GPU tracer, Cornell 1200^2, 2048 spp, depth 50)" if args.scene == "cornell555"
+                   else f"Msamples/s (plain GPU tracer, {args.scene} {W}x{H}, {spp} spp, depth {depth})"),
         "value": round(value, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -456,7 +457,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (procedural scene: the 555-unit Cornell box built through the scene API)",
+        "data": ("synthetic (procedural scene: the 555-unit Cornell box built through the scene API)"
+                 if args.scene == "cornell555" else f"synthetic (procedural scene {args.scene}, built through the scene API)"),
         "config": {"workload": f"{args.scene} {W}x{H}, {spp} spp, max depth {depth}, plain estimator",
                    "width": W, "height": H, "spp": spp, "max_depth": depth},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
