@@ -21,7 +21,8 @@
 // Compiled with g++ -O2 -ffp-contract=off (no fast-math), like the reference.
 //
 // Pinned by: the reference's own code compiled from /root/reference
-// (oracle/_ref, bit-exact hits, camera fields, BVH topology), closed-form
+// (oracle/_ref, bit-exact hits, camera fields, BVH topology, Perlin noise
+// with the reference's tables installed), closed-form
 // and special-case tests, and brute-force cross-checks (tests/test_oracle_*.py).
 #ifndef TPT_ORACLE_ORACLE_H
 #define TPT_ORACLE_ORACLE_H
@@ -35,6 +36,7 @@
 #include "philox.h"
 
 class material;
+class texture;
 
 namespace oracle {
 
@@ -88,6 +90,13 @@ ray camera_ray(const camera_with_blur &cam, float s, float t, rng &R);
 // Returns false when the path would end there.  Used to build secondary rays.
 bool scatter_once(const ray &r, const hit_record &rec, const graph_info &g, const hitable *lights, rng &R,
                   ray *out);
+
+// perlin_noise::noise and perlin_noise::turb (src/utils.cc:160-225) over the
+// scene API's static tables, and texture::value for any texture
+// (src/texture.cc).
+float perlin_noise_at(const vec3 &p);
+float perlin_turb(const vec3 &p, int depth);
+vec3 texture_value(const texture *t, float u, float v, const vec3 &p);
 
 }  // namespace oracle
 

@@ -219,6 +219,10 @@ vec3 sky(const ray &r, float scale) {
 
 }  // namespace
 
+float perlin_noise_at(const vec3 &p) { return perlin(p); }
+float perlin_turb(const vec3 &p, int depth) { return turb(p, depth); }
+vec3 texture_value(const texture *t, float u, float v, const vec3 &p) { return tex_value(t, u, v, p); }
+
 ray camera_ray(const camera_with_blur &cam, float s, float t, rng &R) {
   const vec3 rd = cam.lens_radius_ * in_unit_disk(R);
   const vec3 offset = cam.u_ * rd.x() + cam.v_ * rd.y();

@@ -361,3 +361,43 @@ int ref_render(void *sp, const float *camp, uint32_t w, uint32_t h, uint32_t spp
 }
 
 }  // extern "C"
+
+// ---- the reference's Perlin noise (src/perlin_noise.cc, src/utils.cc:155-225) ----
+extern "C" {
+
+// The reference's current static tables: grad[256][3], perm[3][256].
+void ref_perlin_tables(float *grad, int32_t *perm) {
+  for (int i = 0; i < 256; ++i) {
+    for (int j = 0; j < 3; ++j) grad[3 * i + j] = perlin_noise::random_vec3_[i][j];
+    perm[i] = perlin_noise::permute_x_[i];
+    perm[256 + i] = perlin_noise::permute_y_[i];
+    perm[512 + i] = perlin_noise::permute_z_[i];
+  }
+}
+
+// Constructs the reference's perlin_noise_texture(scale) (its constructor
+// reshuffles the permutations from the clock), installs the given tables in
+// its static arrays, then evaluates noise(p), turb(p) and value(0, 0, p).x()
+// with the reference's own member functions at n points pts[n][3].
+int ref_perlin_eval(const float *grad, const int32_t *perm, const float *pts, uint64_t n, float scale, float *noise,
+                    float *turb, float *tex) {
+  std::thread t([&] {
+    perlin_noise_texture texture(scale);
+    for (int i = 0; i < 256; ++i) {
+      perlin_noise::random_vec3_[i] = vec3(grad[3 * i], grad[3 * i + 1], grad[3 * i + 2]);
+      perlin_noise::permute_x_[i] = perm[i];
+      perlin_noise::permute_y_[i] = perm[256 + i];
+      perlin_noise::permute_z_[i] = perm[512 + i];
+    }
+    for (uint64_t k = 0; k < n; ++k) {
+      const vec3 p(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+      noise[k] = texture.noise_.noise(p);
+      turb[k] = texture.noise_.turb(p);
+      tex[k] = texture.value(0.0f, 0.0f, p).x();
+    }
+  });
+  t.join();
+  return 0;
+}
+
+}  // extern "C"

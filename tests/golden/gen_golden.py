@@ -6,7 +6,8 @@ unmodified sources).  For each scene it stores seeded rays of the four kinds
 (camera primaries, one-bounce secondaries, uniform, adversarial) with the
 reference's hitable::hit result: leaf id, t, the hit_record (u, v, point,
 normal) and the material index; plus camera fields and pre-order bvh_node
-boxes.  The fixtures are data only (inputs and expected outputs).
+boxes; and the Perlin noise vectors of ref_perlin.npz.  The fixtures are data
+only (inputs and expected outputs).
 
     python tests/golden/gen_golden.py
 """
@@ -56,6 +57,26 @@ def main():
         out[f"camera/{name}/fields"] = R.camera_fields(p)
     path = os.path.join(HERE, "ref_hits.npz")
     np.savez_compressed(path, **out)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+    perlin(O, R)
+
+
+def perlin(O, R):
+    """ref_perlin.npz: the reference's perlin_noise gradient table as its
+    constructor draws it on a fresh thread, and its noise(p), turb(p) and
+    perlin_noise_texture(2).value(0, 0, p).x() at 2048 points with that table
+    and the scene API's permutations installed (the reference's own
+    permutations come from the clock)."""
+    O.scene("ref_two_perlin_spheres")
+    R.scene("ref_two_perlin_spheres")
+    grad, _ = R.perlin_tables()
+    _, perm = O.perlin_tables()
+    rng = np.random.default_rng(2025)
+    pts = np.concatenate([rng.uniform(-4, 4, (1024, 3)), rng.uniform(-700, 700, (768, 3)),
+                          np.round(rng.uniform(-300, 300, (256, 3)))]).astype(np.float32)
+    noise, turb, tex = R.perlin_eval(grad, perm, pts, 2.0)
+    path = os.path.join(HERE, "ref_perlin.npz")
+    np.savez_compressed(path, grad=grad, perm=perm, pts=pts, scale=np.float32(2.0), noise=noise, turb=turb, tex=tex)
     print(f"wrote {path} ({os.path.getsize(path)} bytes)")
 
 

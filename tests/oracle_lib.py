@@ -26,6 +26,20 @@ def _nthreads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
+def _perlin_eval(fn, grad, perm, pts, scale, octaves):
+    grad = np.ascontiguousarray(grad, np.float32)
+    perm = np.ascontiguousarray(perm, np.int32)
+    pts = np.ascontiguousarray(pts, np.float32)
+    assert grad.size == 768 and perm.size == 768 and pts.ndim == 2 and pts.shape[1] == 3
+    n = pts.shape[0]
+    noise, turb, tex = (np.zeros(n, np.float32) for _ in range(3))
+    args = [grad.ctypes.data, perm.ctypes.data, pts.ctypes.data, C.c_uint64(n), C.c_float(scale)]
+    if octaves is not None:
+        args.append(C.c_int(octaves))
+    fn(*args, noise.ctypes.data, turb.ctypes.data, tex.ctypes.data)
+    return noise, turb, tex
+
+
 class Oracle:
     def __init__(self, path=ORACLE_SO):
         if not os.path.exists(path):
@@ -33,6 +47,8 @@ class Oracle:
         L = C.CDLL(path)
         self.L = L
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_perlin_tables.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_perlin_eval.argtypes = [C.c_void_p] * 3 + [C.c_uint64, C.c_float, C.c_int] + [C.c_void_p] * 3
         L.oracle_scene_build.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p)]
         L.oracle_scene_destroy.argtypes = [C.c_void_p]
         L.oracle_leaf_count.argtypes = [C.c_void_p]
@@ -49,6 +65,16 @@ class Oracle:
         L.oracle_prim_hit.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_float, C.c_float, _f32p, _f32p]
         L.oracle_camera_from_params.argtypes = [_f32p, _f32p]
         L.oracle_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
+
+    def perlin_tables(self):
+        """The scene API's current Perlin tables: grad (256, 3), perm (3, 256)."""
+        grad, perm = np.zeros(768, np.float32), np.zeros(768, np.int32)
+        self.L.oracle_perlin_tables(grad.ctypes.data, perm.ctypes.data)
+        return grad.reshape(256, 3), perm.reshape(3, 256)
+
+    def perlin_eval(self, grad, perm, pts, scale, octaves=5):
+        """noise(p), turb(p, octaves), perlin_noise_texture(scale).value(0, 0, p).x() with the given tables."""
+        return _perlin_eval(self.L.oracle_perlin_eval, grad, perm, pts, scale, octaves)
 
     def philox(self, ctr, key):
         out = np.zeros(4, np.uint32)
@@ -148,6 +174,8 @@ class Reference:
         L = C.CDLL(path)
         self.L = L
         L.ref_scene_build.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.ref_perlin_tables.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_perlin_eval.argtypes = [C.c_void_p] * 3 + [C.c_uint64, C.c_float] + [C.c_void_p] * 3
         L.ref_leaf_count.argtypes = [C.c_void_p]
         L.ref_trace.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_float, C.c_float, _i32p, _f32p,
                                 C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
@@ -182,6 +210,16 @@ class Reference:
         out = np.zeros(24, np.float32)
         self.L.ref_camera_fields(np.asarray(params, np.float32), out)
         return out
+
+    def perlin_tables(self):
+        """The reference's current perlin_noise static tables: grad (256, 3), perm (3, 256)."""
+        grad, perm = np.zeros(768, np.float32), np.zeros(768, np.int32)
+        self.L.ref_perlin_tables(grad.ctypes.data, perm.ctypes.data)
+        return grad.reshape(256, 3), perm.reshape(3, 256)
+
+    def perlin_eval(self, grad, perm, pts, scale):
+        """The reference's noise(p), turb(p) and perlin_noise_texture(scale).value(0, 0, p).x(), given tables."""
+        return _perlin_eval(self.L.ref_perlin_eval, grad, perm, pts, scale, None)
 
     def bvh_dump(self, h, cap=1 << 20):
         out = np.zeros(6 * cap, np.float32)
