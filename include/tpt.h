@@ -326,7 +326,7 @@ typedef struct tpt_render_desc {
   uint32_t sample_offset; /* first sample index of this pass */
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
-  uint32_t queue_capacity; /* paths in flight; 0 = automatic (2^26, at most the render's samples) */
+  uint32_t queue_capacity; /* paths in flight, at most 2^26; 0 = automatic (2^26, at most the render's samples) */
   uint32_t flags;          /* TPT_RENDER_* flags or 0 */
   uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
                               the rest with the reference sampler (0 = 2); ignored by tpt_render */
@@ -392,7 +392,7 @@ typedef struct tpt_job {
 
 /* Launch settings of one tpt_render_jobs call. */
 typedef struct tpt_launch {
-  uint32_t queue_capacity;  /* path records in flight; 0 = 2^26 */
+  uint32_t queue_capacity;  /* path records in flight, at most 2^26; 0 = 2^26 */
   uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 32 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
                                even); 0 = one launch at a time */

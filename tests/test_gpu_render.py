@@ -33,7 +33,8 @@ def zscores(s1, q1, n1, s2, q2, n2):
 
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 48, 32), ("random_spheres", 48, 16), ("ref_cornell_box", 40, 16),
                                         ("random_spheres_textured", 40, 8), ("box_list", 40, 8),
-                                        ("ref_random_scene", 48, 16), ("cornell_smoke", 40, 16)])
+                                        ("ref_random_scene", 48, 16), ("cornell_smoke", 40, 16),
+                                        ("ref_two_perlin_spheres", 40, 8), ("ref_two_checker_spheres", 40, 8)])
 def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
     s_g, q_g, st = gpu_render(tpt, name, w, w, spp, seed=11)
     s_c, q_c = oracle.scene(name).render(w, w, spp, seed=11)
@@ -47,7 +48,8 @@ def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
 
 
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 40, 64), ("random_spheres", 40, 32),
-                                        ("ref_random_scene", 40, 32), ("cornell_smoke", 40, 64)])
+                                        ("ref_random_scene", 40, 32), ("cornell_smoke", 40, 64),
+                                        ("ref_two_perlin_spheres", 40, 32)])
 def test_independent_streams_within_3_sigma(tpt, oracle, device, name, w, spp):
     s_g, q_g, _ = gpu_render(tpt, name, w, w, spp, seed=123)
     s_c, q_c = oracle.scene(name).render(w, w, spp, seed=456)
