@@ -326,7 +326,7 @@ typedef struct tpt_render_desc {
   uint32_t sample_offset; /* first sample index of this pass */
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
-  uint32_t queue_capacity; /* paths in flight, at most 2^26; 0 = automatic (2^26, at most the render's samples) */
+  uint32_t queue_capacity; /* paths in flight, at most 2^27; 0 = automatic (2^27, at most the render's samples) */
   uint32_t flags;          /* TPT_RENDER_* flags or 0 */
   uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
                               the rest with the reference sampler (0 = 2); ignored by tpt_render */
@@ -392,7 +392,7 @@ typedef struct tpt_job {
 
 /* Launch settings of one tpt_render_jobs call. */
 typedef struct tpt_launch {
-  uint32_t queue_capacity;  /* path records in flight, at most 2^26; 0 = 2^26 */
+  uint32_t queue_capacity;  /* path records in flight, at most 2^27; 0 = 2^27 */
   uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 32 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
                                even); 0 = one launch at a time */
@@ -408,7 +408,7 @@ int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, cons
                     tpt_guide *guide, tpt_render_stats *stats, void *stream);
 
 /* Autotuner: times calibration renders of the jobs (at most calib_spp
- * samples each, 0 = 2, into scratch sums) over queue capacities (2^23..2^26),
+ * samples each, 0 = 2, into scratch sums) over queue capacities (2^24..2^27),
  * grid sizes (8, 16, 32 blocks per CU) and graph replay lengths (0, 8), and
  * writes the fastest settings into *best (its flags, guiding and
  * guide_vertices are kept).  *best_ms: that calibration's wall time. */

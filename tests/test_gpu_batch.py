@@ -124,7 +124,7 @@ def test_autotune_picks_a_valid_launch(tpt, pool, hosts):
     jobs, _ = batch_jobs(pool, sps)
     best, ms = pool.autotune(jobs, calib_spp=1)
     print("autotune:", best.as_dict(), f"{ms:.2f} ms")
-    assert best.queue_capacity in (1 << 23, 1 << 24, 1 << 25, 1 << 26)
+    assert best.queue_capacity in (1 << 24, 1 << 25, 1 << 26, 1 << 27)
     assert best.blocks_per_cu in (8, 16, 32) and best.graph_iters in (0, 8) and ms > 0
     # the tuned launch gives the same sums as the default one
     jobs_a, bufs_a = batch_jobs(pool, sps)

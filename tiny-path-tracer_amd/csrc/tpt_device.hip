@@ -778,7 +778,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL))
     return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
-  if (L.queue_capacity > (1u << 26)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^26");
+  if (L.queue_capacity > (1u << 27)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^27");
   if (L.blocks_per_cu > 64) return tpt::fail(TPT_ERR_INVALID, "tpt_render: blocks_per_cu above 64");
   if (L.graph_iters > 256) return tpt::fail(TPT_ERR_INVALID, "tpt_render: graph_iters above 256");
   uint32_t max_depth = 0;
@@ -858,9 +858,9 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   bool all_simple = true;
   for (uint32_t j = 0; j < n_jobs; ++j) all_simple = all_simple && s->simple[jobs[j].scene];
   const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * guide_shade_blocks_per_cu(all_simple);
-  // 2^26 records in flight by default (19 GB of queues): launches long enough
+  // 2^27 records in flight by default (39 GB of queues): launches long enough
   // that their ramp-up and tail cost little
-  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 26);
+  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 27);
   cap = std::min(cap, max_items);
   if ((rc = ensure_state(s, cap, max_items, grid, guide_grid)) != TPT_OK) return rc;
 
@@ -1198,7 +1198,7 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
     cj[j].d_sum = scratch[j];
     cj[j].d_sumsq = nullptr;
   }
-  const uint32_t caps[] = {1u << 23, 1u << 24, 1u << 25, 1u << 26};
+  const uint32_t caps[] = {1u << 24, 1u << 25, 1u << 26, 1u << 27};
   const uint32_t bpcs[] = {8, 16, 32};
   const uint32_t graphs[] = {0, 8};
   tpt_launch base = *best;
