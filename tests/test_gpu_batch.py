@@ -86,10 +86,10 @@ def test_one_job_of_a_pooled_scene(tpt, pool, singles, hosts):
 
 
 def test_jobs_spanning_several_passes(tpt, pool, singles, hosts):
-    """2^29 items per pass: 1024x1024 at 560 spp needs two passes, split mid-job."""
+    """2^30 items per pass: 1024x1024 at 1100 spp needs two passes, split mid-job."""
     import torch
 
-    sps = [dict(scene=0, width=1024, height=1024, spp=560, seed=3, sample_offset=0, max_depth=50),
+    sps = [dict(scene=0, width=1024, height=1024, spp=1100, seed=3, sample_offset=0, max_depth=50),
            dict(scene=4, width=256, height=256, spp=9, seed=4, sample_offset=2, max_depth=50)]
     jobs, bufs = batch_jobs(pool, sps)
     L = tpt.Launch()

@@ -629,10 +629,10 @@ int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min
 namespace {
 
 // Items per pass.  A pass ends by draining the queues (its last bounces run
-// at low occupancy), so passes are long: 2^29 items hold 6.4 GB of per-sample
+// at low occupancy), so passes are long: 2^30 items hold 12.9 GB of per-sample
 // results.  With training rows recorded a pass is 2^26 items (the rows of one
 // pass are kept until the next).
-constexpr uint64_t kPassItems = 1ull << 29;
+constexpr uint64_t kPassItems = 1ull << 30;
 constexpr uint64_t kPassItemsRecording = 1ull << 26;
 constexpr uint32_t kMaxEnt = 0xFFFFu;        // pass entry index shares a word with the depth
 
