@@ -59,7 +59,7 @@ def parse():
 
 
 RMSE_TARGET = 0.01
-# Online guided schedule: 4 training passes of 4 spp, each recording every 8th
+# Online guided schedule: 3 training passes of 4 spp, each recording every 8th
 # path (2 diffuse vertices) and followed by 64 Adam steps of 2^18 rows at
 # learning rate 5e-3; then passes of 16 spp with the trained guide.
 GUIDED_SCHEDULE = dict(train_passes=3, pass_spp=4, record_stride=8, iters=64, batch=1 << 18, lr=5e-3)

@@ -187,3 +187,57 @@ def test_tonemap_matches_reference_transform(tpt):
     # sqrt(0.25) = 0.5 -> int(255.99*0.5) = 127; 1.5 -> clamp 255; sqrt(.01)=.1 -> 25
     assert lines[3].split() == ["0", "127", "255"]
     assert lines[4].split() == ["0", "255", "25"]
+
+
+def test_sum_buffers_are_validated(tpt):
+    """Caller-supplied accumulators are checked before their pointers reach a
+    kernel: wrong dtype, shape (another resolution), layout or placement raise."""
+    torch = pytest.importorskip("torch")
+    shape = (4, 6, 3)
+    with pytest.raises(tpt.TptError, match="float32"):
+        tpt._check_sums(torch.zeros(shape, dtype=torch.float64), shape, "sum_buf")
+    with pytest.raises(tpt.TptError, match="does not match"):
+        tpt._check_sums(torch.zeros((6, 4, 3)), shape, "sum_buf")
+    with pytest.raises(tpt.TptError, match="does not match"):
+        tpt._check_sums(torch.zeros(4 * 6 * 3), shape, "sum_buf")
+    with pytest.raises(tpt.TptError, match="contiguous"):
+        tpt._check_sums(torch.zeros((6, 4, 3)).transpose(0, 1), shape, "sum_buf")
+    with pytest.raises(tpt.TptError, match="device"):
+        tpt._check_sums(torch.zeros(shape), shape, "sum_buf")
+    with pytest.raises(tpt.TptError, match="torch tensor"):
+        tpt._check_sums(np.zeros(shape, np.float32), shape, "sum_buf")
+
+
+def _jobs(tpt, sizes):
+    out = []
+    for w, h, spp in sizes:
+        j = tpt.Job()
+        j.width, j.height, j.spp, j.max_depth = w, h, spp, 50
+        out.append(j)
+    return out
+
+
+def test_autotune_plan_times_distinct_filled_capacities(tpt):
+    """The autotuner's queue-capacity candidates are distinct after clipping to
+    the real render's largest pass, and its calibration pass holds at least
+    twice the largest of them (or is the whole render)."""
+    import bench
+
+    specs = bench.batch64_specs()
+    jobs = _jobs(tpt, [(sp["res"], sp["res"], sp["spp"]) for sp in specs])
+    caps, k = tpt.autotune_plan(jobs, calib_spp=1)
+    assert caps == [1 << 24, 1 << 25, 1 << 26, 1 << 27]
+    calib_items = sum(sp["res"] ** 2 * min(sp["spp"], k) for sp in specs)
+    assert calib_items >= 2 * caps[-1]
+    assert sum(sp["res"] ** 2 * min(sp["spp"], k // 2) for sp in specs) < 2 * caps[-1]  # the smallest such cap
+
+    # a small workload: every candidate at or above its size collapses to one
+    small = _jobs(tpt, [(64, 64, 8), (128, 128, 4)])
+    caps, k = tpt.autotune_plan(small, calib_spp=1)
+    assert caps == [64 * 64 * 8 + 128 * 128 * 4] and k == 8
+    # recording passes are 2^26 items: the 2^27 candidate collapses into it
+    L = tpt.Launch()
+    L.guiding = 8
+    caps, _ = tpt.autotune_plan(_jobs(tpt, [(1024, 1024, 256)]), base=L, calib_spp=1)
+    assert caps == [1 << 24, 1 << 25, 1 << 26]
+    assert len(set(caps)) == len(caps)

@@ -115,3 +115,115 @@ def test_bad_files_are_refused(tpt, tmp_path):
     junk.write_bytes(b"hello")
     with pytest.raises(tpt.TptError, match="not a JPEG"):
         load(tpt, junk)
+
+
+@pytest.mark.parametrize("header", [b"P6 6148914691236517206 1 255\n", b"P6 1 6148914691236517206 255\n",
+                                    b"P6 70000 1 255\n", b"P6 20000 20000 255\n",
+                                    b"P3 99999999999999999999999 2 255\n"])
+def test_oversized_ppm_headers_are_refused(tpt, tmp_path, header):
+    """Sizes whose pixel count would wrap, or exceed the loader's bound
+    (65535 per side, 2^28 pixels), are rejected while parsing the header."""
+    p = tmp_path / "big.ppm"
+    p.write_bytes(header + b"\x00" * 64)
+    with pytest.raises(tpt.TptError, match="PPM"):
+        load(tpt, p)
+
+
+def jpeg_segments(data):
+    """(marker, payload offset, payload length) of every header segment before
+    the first scan's entropy-coded data."""
+    out, pos = [], 2
+    while pos + 4 <= len(data):
+        assert data[pos] == 0xFF
+        m = data[pos + 1]
+        n = (data[pos + 2] << 8) | data[pos + 3]
+        out.append((m, pos + 4, n - 2))
+        if m == 0xDA:
+            break
+        pos += 2 + n
+    return out
+
+
+def crafted_jpeg(tmp_path, dc_value=None, ac_value=None):
+    """A valid baseline JPEG whose Huffman tables then map every code of the
+    DC (or AC) tables to an out-of-range magnitude category."""
+    Image = pytest.importorskip("PIL.Image")
+    path = tmp_path / "c.jpg"
+    Image.fromarray(smooth_image(16, 16, 3)).save(path, "JPEG", quality=90, subsampling=0)
+    data = bytearray(path.read_bytes())
+    for m, off, n in jpeg_segments(data):
+        if m != 0xC4:
+            continue
+        p = off
+        while p < off + n:
+            tc = data[p] >> 4
+            counts = data[p + 1:p + 17]
+            total = sum(counts)
+            vals = range(p + 17, p + 17 + total)
+            for v in vals:
+                if tc == 0 and dc_value is not None:
+                    data[v] = dc_value
+                if tc == 1 and ac_value is not None and data[v] != 0x00:
+                    data[v] = ac_value
+            p += 17 + total
+    out = tmp_path / "crafted.jpg"
+    out.write_bytes(bytes(data))
+    return out
+
+
+@pytest.mark.parametrize("dc", [12, 31, 40, 255])
+def test_crafted_dc_category_is_refused(tpt, tmp_path, dc):
+    with pytest.raises(tpt.TptError, match="DC magnitude"):
+        load(tpt, crafted_jpeg(tmp_path, dc_value=dc))
+
+
+def test_crafted_ac_category_is_refused(tpt, tmp_path):
+    with pytest.raises(tpt.TptError, match="AC magnitude"):
+        load(tpt, crafted_jpeg(tmp_path, ac_value=0x0F))
+
+
+def test_oversized_jpeg_frame_is_refused(tpt, tmp_path):
+    Image = pytest.importorskip("PIL.Image")
+    path = tmp_path / "f.jpg"
+    Image.fromarray(smooth_image(8, 8, 4)).save(path, "JPEG", quality=90, subsampling=0)
+    data = bytearray(path.read_bytes())
+    for m, off, n in jpeg_segments(data):
+        if m == 0xC0:
+            data[off + 1:off + 5] = b"\xff\xff\xff\xff"  # 65535 x 65535
+    bad = tmp_path / "huge.jpg"
+    bad.write_bytes(bytes(data))
+    with pytest.raises(tpt.TptError, match="too large"):
+        load(tpt, bad)
+
+
+def test_mutated_files_fail_cleanly(tpt, tmp_path):
+    """Seeded byte mutations of a small JPEG and PPM: each either decodes to an
+    image of the size it reports or is refused with an error; none crashes."""
+    Image = pytest.importorskip("PIL.Image")
+    src = tmp_path / "s.jpg"
+    Image.fromarray(smooth_image(24, 40, 9)).save(src, "JPEG", quality=85, subsampling=2, restart_marker_blocks=2)
+    base = [src.read_bytes(), b"P6\n5 7\n255\n" + smooth_image(7, 5, 2).tobytes()]
+    rng = np.random.default_rng(77)
+    decoded = refused = 0
+    for it in range(600):
+        data = bytearray(base[it % 2])
+        for _ in range(int(rng.integers(1, 6))):
+            k = int(rng.integers(0, len(data)))
+            op = int(rng.integers(0, 3))
+            if op == 0:
+                data[k] = int(rng.integers(0, 256))
+            elif op == 1:
+                del data[k:]
+                if len(data) < 2:
+                    data = bytearray(base[it % 2][:2])
+            else:
+                data.insert(k, int(rng.integers(0, 256)))
+        p = tmp_path / f"m{it % 2}.bin"
+        p.write_bytes(bytes(data))
+        try:
+            img = load(tpt, p)
+            assert img.ndim == 3 and img.shape[2] == 3
+            decoded += 1
+        except tpt.TptError:
+            refused += 1
+    assert decoded + refused == 600 and refused > 0

@@ -20,7 +20,7 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "HostScene", "Scene", "Guide", "TptError", "scene_names", "build", "tonemap_ppm"]
+__all__ = ["lib", "HostScene", "Scene", "Guide", "TptError", "scene_names", "build", "tonemap_ppm", "autotune_plan"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtpt.so")
@@ -142,6 +142,8 @@ def lib():
     L.tpt_scene_create_pool.argtypes = [C.POINTER(FlatScene), u32, C.POINTER(vp)]
     L.tpt_render_jobs.argtypes = [vp, C.POINTER(Job), u32, C.POINTER(Launch), vp, C.POINTER(RenderStats), vp]
     L.tpt_autotune.argtypes = [vp, C.POINTER(Job), u32, vp, u32, C.POINTER(Launch), C.POINTER(C.c_float)]
+    L.tpt_autotune_plan.argtypes = [C.POINTER(Job), u32, C.POINTER(Launch), u32, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.tpt_guide_create.argtypes = [u64, C.POINTER(vp)]
     L.tpt_guide_destroy.argtypes = [vp]
     L.tpt_guide_get_params.argtypes = [vp, vp]
@@ -197,6 +199,22 @@ def _torch():
     return torch
 
 
+def _check_sums(buf, shape, what):
+    """A caller-supplied accumulator must be what the kernels index: a
+    contiguous float32 device tensor of shape (H, W, 3)."""
+    torch = _torch()
+    if not isinstance(buf, torch.Tensor):
+        raise TptError(f"{what}: expected a torch tensor, got {type(buf).__name__}")
+    if buf.dtype != torch.float32:
+        raise TptError(f"{what}: dtype must be float32, not {buf.dtype}")
+    if tuple(buf.shape) != tuple(shape):
+        raise TptError(f"{what}: shape {tuple(buf.shape)} does not match the image {tuple(shape)}")
+    if not buf.is_contiguous():
+        raise TptError(f"{what}: must be contiguous")
+    if not buf.is_cuda:
+        raise TptError(f"{what}: must live on the device")
+
+
 class Scene:
     """A scene (or a pool of scenes) uploaded to the current HIP device."""
 
@@ -244,6 +262,9 @@ class Scene:
             sum_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
         if sumsq_buf is None:
             sumsq_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        _check_sums(sum_buf, shape, "sum_buf")
+        if sumsq_buf is not False:
+            _check_sums(sumsq_buf, shape, "sumsq_buf")
         j.d_sum = sum_buf.data_ptr()
         j.d_sumsq = sumsq_buf.data_ptr() if sumsq_buf is not False else None
         return j, sum_buf, sumsq_buf
@@ -335,6 +356,8 @@ class Scene:
             sum_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
         if sumsq_buf is None:
             sumsq_buf = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        _check_sums(sum_buf, shape, "sum_buf")
+        _check_sums(sumsq_buf, shape, "sumsq_buf")
         stats = RenderStats()
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         cam = camera if camera is not None else self.host.camera
@@ -469,6 +492,17 @@ class ProgressiveRender:
 
     def mean(self):
         return self.sum / max(self.spp, 1)
+
+
+def autotune_plan(jobs, base=None, calib_spp=2):
+    """tpt_autotune_plan (host only): the distinct queue capacities the
+    autotuner times and the spp cap of its calibration jobs."""
+    arr = (Job * len(jobs))(*jobs)
+    caps = (C.c_uint32 * 4)()
+    n, k = C.c_uint32(), C.c_uint32()
+    _check(lib().tpt_autotune_plan(arr, len(jobs), C.byref(base) if base is not None else None, calib_spp, caps,
+                                   C.byref(n), C.byref(k)), "tpt_autotune_plan")
+    return list(caps[:n.value]), k.value
 
 
 def display_rmse(sum_buf, spp, reference_display):

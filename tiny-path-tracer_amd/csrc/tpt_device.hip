@@ -1179,10 +1179,51 @@ int tpt_render_jobs(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tp
   return render_core(s, jobs, n_jobs, L, guide, stats, stream);
 }
 
+int tpt_autotune_plan(const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *base, uint32_t calib_spp,
+                      uint32_t caps[4], uint32_t *n_caps, uint32_t *calib_spp_out) {
+  if (!jobs || n_jobs == 0 || !caps || !n_caps || !calib_spp_out)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_autotune_plan: bad argument");
+  for (uint32_t j = 0; j < n_jobs; ++j)
+    if (jobs[j].width == 0 || jobs[j].height == 0 ||
+        static_cast<uint64_t>(jobs[j].width) * jobs[j].height > kPassItemsRecording)
+      return tpt::fail(TPT_ERR_INVALID, "tpt_autotune_plan: bad image size");
+  auto largest_pass = [&](const std::vector<tpt_job> &js, uint64_t pass_items) {
+    uint64_t m = 0;
+    for (const pass_plan &pp : plan_passes(js.data(), static_cast<uint32_t>(js.size()), pass_items))
+      m = std::max<uint64_t>(m, pp.n_items);
+    return m;
+  };
+  const std::vector<tpt_job> real(jobs, jobs + n_jobs);
+  const bool recording = base && base->guiding;
+  const uint64_t real_max = largest_pass(real, recording ? kPassItemsRecording : kPassItems);
+  uint32_t n = 0;
+  for (uint32_t c = 1u << 24; c <= (1u << 27); c <<= 1) {
+    const uint32_t eff = static_cast<uint32_t>(std::min<uint64_t>(c, real_max));
+    bool dup = false;
+    for (uint32_t k = 0; k < n; ++k) dup = dup || caps[k] == eff;
+    if (!dup) caps[n++] = eff;
+  }
+  *n_caps = n;
+  uint32_t max_spp = 0;
+  for (uint32_t j = 0; j < n_jobs; ++j) max_spp = std::max(max_spp, jobs[j].spp);
+  const uint64_t want = 2ull * caps[n - 1];
+  uint32_t k = calib_spp ? calib_spp : 2;
+  std::vector<tpt_job> cj(real);
+  for (;;) {
+    for (uint32_t j = 0; j < n_jobs; ++j) cj[j].spp = std::min(jobs[j].spp, k);
+    if (k >= max_spp || largest_pass(cj, kPassItems) >= want) break;
+    k = k > 0x7FFFFFFFu ? max_spp : 2 * k;
+  }
+  *calib_spp_out = std::min(k, max_spp);
+  return TPT_OK;
+}
+
 int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *guide, uint32_t calib_spp,
                  tpt_launch *best, float *best_ms) {
   if (!s || !jobs || n_jobs == 0 || !best) return tpt::fail(TPT_ERR_INVALID, "tpt_autotune: bad argument");
-  const uint32_t cs_ = calib_spp ? calib_spp : 2;
+  uint32_t caps[4], n_caps = 0, cs_ = 0;
+  int rc = tpt_autotune_plan(jobs, n_jobs, best, calib_spp, caps, &n_caps, &cs_);
+  if (rc != TPT_OK) return rc;
   // calibration jobs: at most cs_ samples each, into scratch sums
   std::vector<tpt_job> cj(jobs, jobs + n_jobs);
   std::vector<float *> scratch(n_jobs, nullptr);
@@ -1198,37 +1239,36 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
     cj[j].d_sum = scratch[j];
     cj[j].d_sumsq = nullptr;
   }
-  const uint32_t caps[] = {1u << 24, 1u << 25, 1u << 26, 1u << 27};
-  const uint32_t bpcs[] = {8, 16, 32};
-  const uint32_t graphs[] = {0, 8};
   tpt_launch base = *best;
   base.flags = 0;
+  base.guiding = 0;  // calibration renders do not record training rows
   float best_t = 3.0e38f;
   tpt_launch pick = base;
-  for (uint32_t g : graphs)
-    for (uint32_t c : caps)
-      for (uint32_t b : bpcs) {
-        tpt_launch L = base;
-        L.queue_capacity = c;
-        L.blocks_per_cu = b;
-        L.graph_iters = g;
-        L.guiding = 0;  // calibration renders do not record training rows
-        float t = 0.0f;
-        for (int rep = 0; rep < 2; ++rep) {  // the first run warms up (state growth, graph capture)
-          const auto t0 = std::chrono::steady_clock::now();
-          const int rc = render_core(s, cj.data(), n_jobs, L, guide, nullptr, nullptr);
-          if (rc != TPT_OK) {
-            release();
-            return rc;
-          }
-          t = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        }
-        if (t < best_t) {
-          best_t = t;
-          pick = L;
-        }
-      }
+  auto trial = [&](uint32_t cap, uint32_t bpc, uint32_t graph) -> int {
+    tpt_launch L = base;
+    L.queue_capacity = cap;
+    L.blocks_per_cu = bpc;
+    L.graph_iters = graph;
+    float t = 0.0f;
+    for (int r = 0; r < 2; ++r) {  // the first run warms up (state growth, graph capture)
+      const auto t0 = std::chrono::steady_clock::now();
+      const int e = render_core(s, cj.data(), n_jobs, L, guide, nullptr, nullptr);
+      if (e != TPT_OK) return e;
+      t = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (t < best_t) {
+      best_t = t;
+      pick = L;
+    }
+    return TPT_OK;
+  };
+  for (uint32_t k = 0; k < n_caps && rc == TPT_OK; ++k) rc = trial(caps[k], 32, 0);
+  const uint32_t cap_pick = pick.queue_capacity;
+  for (uint32_t b : {8u, 16u})
+    if (rc == TPT_OK) rc = trial(cap_pick, b, 0);
+  if (rc == TPT_OK) rc = trial(cap_pick, pick.blocks_per_cu, 8);
   release();
+  if (rc != TPT_OK) return rc;
   pick.flags = best->flags;
   pick.guiding = best->guiding;
   pick.guide_vertices = best->guide_vertices;

@@ -98,8 +98,10 @@ int tpt_image_load(const char *path, uint8_t **out, int *width, int *height) {
   std::vector<uint8_t> rgb;
   std::string err;
   if (!tpt::load_image_rgb8(path, &rgb, width, height, &err)) return tpt::fail(TPT_ERR_INVALID, err);
+  if (rgb.size() != static_cast<size_t>(*width) * static_cast<size_t>(*height) * 3)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_image_load: decoded size does not match the header");
   *out = static_cast<uint8_t *>(std::malloc(rgb.size()));
-  if (!*out) return tpt::fail(TPT_ERR_INVALID, "tpt_image_load: out of memory");
+  if (!*out) return tpt::fail(TPT_ERR_NOMEM, "tpt_image_load: out of memory");
   std::memcpy(*out, rgb.data(), rgb.size());
   return TPT_OK;
 }

@@ -17,9 +17,13 @@
 #include <cstring>
 #include <fstream>
 #include <iterator>
+#include <stdexcept>
 
 namespace tpt {
 namespace {
+
+// Largest image either decoder accepts (2^28 pixels, 768 MiB of RGB8).
+constexpr uint64_t kMaxPixels = 1ull << 28;
 
 // Zig-zag order: natural (row-major) index of the k-th coefficient.
 constexpr int kZigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
@@ -37,7 +41,7 @@ struct component {
   int id = 0, h = 1, v = 1, tq = 0;
   int td = 0, ta = 0;  // Huffman tables of the current scan
   int bw = 0, bh = 0;  // blocks across / down in the component's plane (whole MCUs)
-  int dc_pred = 0;
+  long long dc_pred = 0;  // a crafted file may sum many large differences
   std::vector<uint8_t> plane;  // bw*8 x bh*8 samples
 };
 
@@ -156,6 +160,7 @@ class decoder {
     width_ = u16();
     const int nf = u8();
     if (width_ <= 0 || height_ <= 0) fail("bad image size");
+    if (static_cast<uint64_t>(width_) * static_cast<uint64_t>(height_) > kMaxPixels) fail("image too large");
     if (nf != 1 && nf != 3) fail("only 1 or 3 components are supported");
     comps_.assign(nf, component{});
     for (auto &c : comps_) {
@@ -252,6 +257,7 @@ class decoder {
     double coef[64] = {0};
     const uint16_t *q = q_[c.tq];
     const int t = decode(dc_[c.td]);
+    if (t > 11) fail("bad DC magnitude category");  // F.1.2.1.1: at most 11 for 8-bit samples
     c.dc_pred += extend(receive(t), t);
     coef[0] = static_cast<double>(c.dc_pred) * q[0];
     for (int k = 1; k < 64;) {
@@ -262,6 +268,7 @@ class decoder {
         k += 16;             // ZRL
         continue;
       }
+      if (s > 10) fail("bad AC magnitude category");  // F.1.2.2.1: at most 10 for 8-bit samples
       k += r;
       if (k > 63) fail("bad AC run");
       coef[kZigzag[k]] = static_cast<double>(extend(receive(s), s)) * q[k];
@@ -387,13 +394,20 @@ bool decode_ppm(const std::vector<uint8_t> &f, std::vector<uint8_t> *rgb, int *w
     }
     if (p >= f.size() || !std::isdigit(f[p])) return false;
     long x = 0;
-    while (p < f.size() && std::isdigit(f[p])) x = x * 10 + (f[p++] - '0');
+    while (p < f.size() && std::isdigit(f[p])) {
+      x = x * 10 + (f[p++] - '0');
+      if (x > 1000000) return false;  // no field of an accepted file is this large
+    }
     *v = x;
     return true;
   };
   long W, H, M;
   if (!token(&W) || !token(&H) || !token(&M) || W <= 0 || H <= 0 || M != 255) {
     *err = "bad PPM header (8-bit P3/P6 only)";
+    return false;
+  }
+  if (W > 65535 || H > 65535 || static_cast<uint64_t>(W) * static_cast<uint64_t>(H) > kMaxPixels) {
+    *err = "PPM image too large";
     return false;
   }
   const size_t n = static_cast<size_t>(W) * H * 3;
@@ -428,16 +442,21 @@ bool load_image_rgb8(const std::string &path, std::vector<uint8_t> *rgb, int *wi
     *err = "cannot open " + path;
     return false;
   }
-  if (f.size() >= 2 && f[0] == 0xFF && f[1] == 0xD8) {
-    try {
-      decoder(f.data(), f.size()).run(rgb, width, height);
-      return true;
-    } catch (const jpeg_error &e) {
-      *err = path + ": " + e.what;
-      return false;
+  try {
+    if (f.size() >= 2 && f[0] == 0xFF && f[1] == 0xD8) {
+      try {
+        decoder(f.data(), f.size()).run(rgb, width, height);
+        return true;
+      } catch (const jpeg_error &e) {
+        *err = path + ": " + e.what;
+        return false;
+      }
     }
+    if (f.size() >= 2 && f[0] == 'P' && (f[1] == '6' || f[1] == '3')) return decode_ppm(f, rgb, width, height, err);
+  } catch (const std::exception &e) {  // std::bad_alloc, std::length_error
+    *err = path + ": " + e.what();
+    return false;
   }
-  if (f.size() >= 2 && f[0] == 'P' && (f[1] == '6' || f[1] == '3')) return decode_ppm(f, rgb, width, height, err);
   *err = path + ": not a JPEG or PPM file";
   return false;
 }

@@ -14,8 +14,8 @@ class sphere : public hitable {
   material *mat_ptr_ = nullptr;
 };
 
-// Declared for source compatibility (motion blur is not on the device path):
-// flattening a graph that contains one fails with TPT_ERR_UNSUPPORTED.
+// Motion blur (headers/sphere.h:23-40): flattened to an MSPHERE slot whose
+// centre the device evaluates at the ray's time with center() below.
 class moving_sphere : public hitable {
  public:
   moving_sphere() {}
