@@ -365,6 +365,28 @@ typedef struct tpt_render_stats {
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
                float *d_sumsq, tpt_render_stats *stats, void *stream);
 
+/* tpt_render without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code running under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * arguments, the same sums bit for bit and the same stats as tpt_render
+ * (kernel_ms is host wall time), but it calls no hipMalloc / hipFree /
+ * hipHostMalloc / hipEventCreate: the render state (queues, counters,
+ * per-sample results, job and pass tables, statistics) lives in one statically
+ * allocated device array of TPT_NOALLOC_SCRATCH_BYTES, which each GPU that the
+ * library runs on holds until the process ends.  Limits: at most
+ * TPT_NOALLOC_PATHS path records in flight (desc->queue_capacity 0 = that
+ * many), passes of at most TPT_NOALLOC_PASS_ITEMS samples, so an image of at
+ * most that many pixels; larger values are refused with TPT_ERR_INVALID, as
+ * are TPT_RENDER_PROFILE and a nonzero desc->guiding.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialised by the caller.  The scene's device must
+ * be current. */
+#define TPT_NOALLOC_SCRATCH_BYTES (144u << 20)
+#define TPT_NOALLOC_PATHS (1u << 17)
+#define TPT_NOALLOC_PASS_ITEMS (3u << 19)
+int tpt_render_noalloc(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+                       float *d_sumsq, tpt_render_stats *stats, void *stream);
+
 /* ----------------------------------------------------- batched rendering ---
  * No counterpart in the reference, which renders one scene per run: many
  * renders (scenes, resolutions, sample counts) in one launch sequence.

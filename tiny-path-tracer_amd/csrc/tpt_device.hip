@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -685,6 +686,9 @@ int grow_device(T **p, uint32_t *cap, uint32_t need) {
   return TPT_OK;
 }
 
+// Live-count polls of the bounce loop (see render_core).
+constexpr uint64_t kPoll = 16, kPollFirst = 8, kPollTail = 4;
+
 struct bounce_launch {
   scene_view S;
   render_params P;
@@ -945,7 +949,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   HIP_TRY(hipEventRecord(e0, ws));
   const uint32_t n_stat = std::max(grid, guide_grid);  // rows written by block_stats
   hipLaunchKernelGGL(k_zero_stats, dim3(64), dim3(256), 0, ws, s->Q.stats, n_stat * ST_COUNT);
-  const int poll = 16;
+  const int poll = static_cast<int>(kPoll);
   if (profile && s->events.size() < static_cast<size_t>(2 * poll + 1)) {
     for (auto e : s->events) (void)hipEventDestroy(e);
     s->events.assign(2 * poll + 1, nullptr);
@@ -973,9 +977,8 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   // The fused tail (k_tail) takes over a pass once it has handed out every
   // item (fewer live records than the capacity) and at most tail_live paths
   // are left: from there on a bounce launch pair costs mostly latency.  Polls
-  // come every `poll` iterations, the first after kPollFirst, and every
+  // come every kPoll iterations, the first after kPollFirst, and every
   // kPollTail once the items are out, so the switch comes early.
-  constexpr uint64_t kPollFirst = 8, kPollTail = 4;
   const uint64_t tail_live = std::min<uint64_t>(cap / 4, 1u << 18);
   double tail_ms = 0.0;
   uint64_t n_tail = 0;
@@ -1156,9 +1159,285 @@ int render_single(tpt_scene *s, tpt_guide *guide, const tpt_camera *camera, cons
   return render_core(s, &J, 1, L, guide, stats, stream);
 }
 
+// ---------------------------------------------- render without allocation ---
+// All render state of tpt_render_noalloc lives in `scratch`.  In host code the
+// name stands for a host-side shadow, not device memory: the array's device
+// address on each GPU comes from hipGetSymbolAddress (noalloc_base), and only
+// that address is passed to kernels.
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_base[kNoallocMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[TPT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` on device `dev`, which must be current; taken
+// once per device and kept.
+int noalloc_base(int dev, char **out) {
+  if (dev < 0 || dev >= kNoallocMaxDevices)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_base[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return tpt::fail(TPT_ERR_HIP, std::string("tpt_render_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_noalloc_base[dev] = static_cast<char *>(p);
+  }
+  *out = g_noalloc_base[dev];
+  return TPT_OK;
+}
+
+// The job, the pass header and the one pass-table entry of a pass, as kernel
+// arguments: the host needs no staging buffer.
+__global__ void k_noalloc_pass(job_dev *jobs, job_dev job, pass_hdr *hdr, pass_hdr h, pass_ent *pass, pass_ent e) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    jobs[0] = job;
+    *hdr = h;
+    pass[0] = e;
+  }
+}
+
+// Column sums of `rows` per-block statistics rows into out[0..ST_COUNT).
+__global__ void __launch_bounds__(256) k_noalloc_stats(const unsigned long long *stats, uint32_t rows,
+                                                       unsigned long long *out) {
+  __shared__ unsigned long long part[256];
+  constexpr uint32_t kLanes = 256 / ST_COUNT;  // partial sums per counter
+  const uint32_t k = threadIdx.x % ST_COUNT, lane = threadIdx.x / ST_COUNT;
+  unsigned long long v = 0;
+  for (uint32_t r = lane; r < rows; r += kLanes) v += stats[static_cast<size_t>(r) * ST_COUNT + k];
+  part[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < ST_COUNT) {
+    unsigned long long t = 0;
+    for (uint32_t l = 0; l < kLanes; ++l) t += part[l * ST_COUNT + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+// Grid of the no-allocation render: 4 blocks of 256 per CU, at most 1024
+// blocks (the statistics rows and the region bound are sized for that), at
+// least NSUB (the fused tail launches at least NSUB blocks).
+constexpr uint32_t kNoallocMaxGrid = 1024;
+
+// Byte offsets of the render state inside `scratch`, in 256-byte pieces.
+struct noalloc_layout {
+  size_t hdr, pass, jobs, counters, stats, totals, rq[2][3], mq[3][4], res, end;
+  uint32_t region;
+};
+
+noalloc_layout noalloc_plan(uint32_t grid) {
+  noalloc_layout L{};
+  // Plain renders: only the grid-G trace and shade launches of 256-thread
+  // blocks append (k_trace to the material queues, k_shade to the next ray
+  // queue), at most one entry per live record, so a region holds at most
+  // t/NSUB + t/G + ceil(G/NSUB)*256 entries (region_size without the guided
+  // shading term); k_init's runs of 64 fit in the +256.
+  const uint32_t cap = TPT_NOALLOC_PATHS;
+  L.region = cap / NSUB + cap / grid + (grid + NSUB - 1) / NSUB * 256 + 256;
+  const size_t n = static_cast<size_t>(L.region) * NSUB;
+  size_t w = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = w;
+    w += (bytes + 255) & ~size_t(255);
+    return at;
+  };
+  L.hdr = take(sizeof(pass_hdr));
+  L.pass = take(sizeof(pass_ent));
+  L.jobs = take(sizeof(job_dev));
+  L.counters = take(COUNTER_WORDS * sizeof(uint32_t));
+  L.stats = take(static_cast<size_t>(kNoallocMaxGrid) * ST_COUNT * sizeof(unsigned long long));
+  L.totals = take(ST_COUNT * sizeof(unsigned long long));
+  for (int q = 0; q < 2; ++q)
+    for (int k = 0; k < 3; ++k) L.rq[q][k] = take(n * 16);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < 4; ++k) L.mq[c][k] = take(n * 16);
+  L.res = take(static_cast<size_t>(TPT_NOALLOC_PASS_ITEMS) * 3 * sizeof(float));
+  L.end = w;
+  return L;
+}
+
+int render_noalloc(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+                   float *d_sumsq, tpt_render_stats *stats, void *stream_ptr) {
+  if (!s || !camera || !desc || !d_sum) return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: null argument");
+  if (desc->flags & ~(TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL))
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: unsupported flags (profiling needs events)");
+  if (desc->guiding) return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: guiding needs tpt_render_guided");
+  if (desc->queue_capacity > TPT_NOALLOC_PATHS)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: queue_capacity above TPT_NOALLOC_PATHS (2^17)");
+  if (desc->width == 0 || desc->height == 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: empty image");
+  const uint64_t npix = static_cast<uint64_t>(desc->width) * desc->height;
+  if (npix > TPT_NOALLOC_PASS_ITEMS)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: image of " + std::to_string(npix) +
+                                          " pixels above TPT_NOALLOC_PASS_ITEMS (" +
+                                          std::to_string(TPT_NOALLOC_PASS_ITEMS) + ")");
+  if (desc->max_depth > 0xFFFEu) return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: max_depth above 65534");
+  if (static_cast<uint64_t>(desc->sample_offset) + desc->spp > 0xFFFFFFFFull)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: sample range overflows");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (desc->spp == 0) return TPT_OK;
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != s->device) return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: the scene's device is not current");
+  char *base = nullptr;
+  int rc = noalloc_base(dev, &base);
+  if (rc != TPT_OK) return rc;
+  const uint32_t grid = std::max<uint32_t>(NSUB, 4u * static_cast<uint32_t>(std::min(s->n_cu, 256)));
+  const noalloc_layout Lo = noalloc_plan(grid);
+  if (Lo.end > TPT_NOALLOC_SCRATCH_BYTES)
+    return tpt::fail(TPT_ERR_INVALID, "tpt_render_noalloc: render state exceeds the scratch array");
+  const auto t_begin = std::chrono::steady_clock::now();
+  hipStream_t ws = static_cast<hipStream_t>(stream_ptr);
+
+  render_state Q{};
+  Q.region = Lo.region;
+  Q.counters = reinterpret_cast<uint32_t *>(base + Lo.counters);
+  Q.stats = reinterpret_cast<unsigned long long *>(base + Lo.stats);
+  for (int q = 0; q < 2; ++q)
+    for (int k = 0; k < 3; ++k) Q.rq[q][k] = reinterpret_cast<float4 *>(base + Lo.rq[q][k]);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < 4; ++k) Q.mq[c][k] = reinterpret_cast<float4 *>(base + Lo.mq[c][k]);
+  Q.res = reinterpret_cast<float *>(base + Lo.res);
+  job_dev *d_jobs = reinterpret_cast<job_dev *>(base + Lo.jobs);
+  pass_hdr *d_hdr = reinterpret_cast<pass_hdr *>(base + Lo.hdr);
+  pass_ent *d_pass = reinterpret_cast<pass_ent *>(base + Lo.pass);
+  unsigned long long *d_totals = reinterpret_cast<unsigned long long *>(base + Lo.totals);
+
+  job_dev jd;
+  std::memset(&jd, 0, sizeof(jd));
+  jd.cam = *camera;
+  jd.width = desc->width;
+  jd.height = desc->height;
+  jd.n_pix = static_cast<uint32_t>(npix);
+  jd.max_depth = desc->max_depth;
+  jd.seed = desc->seed;
+  jd.background = desc->background;
+  jd.scene = 0;
+  jd.sum = d_sum;
+  jd.sumsq = d_sumsq;
+
+  bounce_launch B;
+  std::memset(&B, 0, sizeof(B));
+  B.S = s->view(0);
+  B.Q = Q;
+  B.grid = grid;
+  B.guide_grid = grid;
+  B.sweep = s->sweep[0];
+  B.sph = s->sph[0];
+  B.simple = s->simple[0];
+  B.multi = false;
+  B.guided = false;
+  B.tail = !(desc->flags & TPT_RENDER_NO_TAIL) && B.simple && (B.sweep || B.sph);
+  render_params &P = B.P;
+  P.jobs = d_jobs;
+  P.scenes = s->d_scenes;
+  P.hdr = d_hdr;
+  P.pass = d_pass;
+  P.guide_verts = ~0u;
+  P.motion = s->motion[0];
+  P.media = s->media[0];
+  P.media_scatter = (desc->flags & TPT_RENDER_MEDIA_SCATTER) ? 1u : 0u;
+
+  const uint32_t layers = static_cast<uint32_t>(TPT_NOALLOC_PASS_ITEMS / npix);
+  const uint32_t first_items = static_cast<uint32_t>(std::min<uint64_t>(layers, desc->spp) * npix);
+  uint32_t cap = desc->queue_capacity ? desc->queue_capacity : TPT_NOALLOC_PATHS;
+  cap = std::min(cap, first_items);
+  const uint64_t tail_live = std::min<uint64_t>(cap / 4, 1u << 18);
+
+  // live records of ray queue p (the scene's pinned count buffer)
+  auto live_count = [&](uint32_t p, uint64_t *live) -> int {
+    HIP_TRY(hipMemcpyAsync(s->host_count, Q.counters + rc_at(p, 0), NSUB * CSTRIDE * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ws));
+    HIP_TRY(hipStreamSynchronize(ws));
+    *live = 0;
+    for (uint32_t k = 0; k < NSUB; ++k) *live += s->host_count[k * CSTRIDE];
+    return TPT_OK;
+  };
+  uint64_t n_tail = 0;
+  hipLaunchKernelGGL(k_zero_stats, dim3(64), dim3(256), 0, ws, Q.stats, kNoallocMaxGrid * ST_COUNT);
+  bool drained = true;
+  const uint32_t s_end = desc->sample_offset + desc->spp;
+  for (uint32_t s0 = desc->sample_offset; s0 < s_end && drained;) {
+    const uint32_t take = std::min(layers, s_end - s0);
+    pass_hdr h{};
+    h.n_items = static_cast<uint32_t>(take * npix);
+    h.n_first = std::min(cap, h.n_items);
+    h.n_ent = 1;
+    h.n_acc = static_cast<uint32_t>(npix);
+    pass_ent e{};
+    e.item_begin = 0;
+    e.job = 0;
+    e.s_begin = s0;
+    e.n_layers = take;
+    e.acc_begin = 0;
+    hipLaunchKernelGGL(k_noalloc_pass, dim3(1), dim3(64), 0, ws, d_jobs, jd, d_hdr, h, d_pass, e);
+    hipLaunchKernelGGL(k_init<false>, dim3(std::min<uint32_t>((h.n_first + 255) / 256 + 1, grid)), dim3(256), 0,
+                       ws, P, Q);
+    const uint64_t max_iter = (static_cast<uint64_t>(h.n_items) / cap + 2) * (desc->max_depth + 2ull) + 64;
+    drained = false;
+    bool exhausted = h.n_items <= cap;
+    uint64_t next_poll = (B.tail ? kPollFirst : kPoll) - 1;
+    for (uint64_t it = 0; it < max_iter; ++it) {
+      const uint32_t parity = static_cast<uint32_t>(it & 1u);
+      launch_trace(B, ws, parity);
+      launch_shade(B, ws, parity);
+      if (it != next_poll) continue;
+      uint64_t live = 0;
+      if ((rc = live_count(parity ^ 1u, &live)) != TPT_OK) return rc;
+      if (live == 0) {
+        drained = true;
+        break;
+      }
+      exhausted = exhausted || live < cap;
+      if (B.tail && exhausted && live <= tail_live) {
+        const uint32_t blocks =
+            static_cast<uint32_t>(std::max<uint64_t>(NSUB, std::min<uint64_t>(grid, (live + 255) / 256)));
+        launch_tail(B, ws, parity ^ 1u, blocks);
+        ++n_tail;
+        drained = true;
+        break;
+      }
+      next_poll = it + (B.tail && exhausted ? kPollTail : kPoll);
+    }
+    if (!drained) break;
+    hipLaunchKernelGGL(k_accumulate<false>, dim3(std::min<uint32_t>((h.n_acc + 255) / 256, grid)), dim3(256), 0, ws,
+                       P, static_cast<const float *>(Q.res));
+    s0 += take;
+  }
+  hipLaunchKernelGGL(k_noalloc_stats, dim3(1), dim3(256), 0, ws, static_cast<const unsigned long long *>(Q.stats),
+                     grid, d_totals);
+  HIP_TRY(hipGetLastError());
+  // the totals come back through the scene's pinned count buffer (8 KB)
+  static_assert(ST_COUNT * sizeof(unsigned long long) <= NSUB * CSTRIDE * sizeof(uint32_t), "pinned buffer size");
+  HIP_TRY(hipMemcpyAsync(s->host_count, d_totals, ST_COUNT * sizeof(unsigned long long), hipMemcpyDeviceToHost, ws));
+  HIP_TRY(hipStreamSynchronize(ws));
+  if (!drained) return tpt::fail(TPT_ERR_HIP, "tpt_render_noalloc: bounce loop did not drain");
+  if (stats) {
+    unsigned long long tot[ST_COUNT];
+    std::memcpy(tot, s->host_count, sizeof(tot));
+    stats->paths = tot[ST_PATHS];
+    stats->segments = tot[ST_SEGMENTS];
+    stats->light_queries = tot[ST_LIGHT];
+    stats->nan_samples = tot[ST_NAN];
+    stats->nodes_visited = tot[ST_NODES];
+    stats->tail_segments = tot[ST_TAIL];
+    stats->tail_launches = n_tail;
+    stats->kernel_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
+  }
+  return TPT_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int tpt_render_noalloc(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,
+                       float *d_sumsq, tpt_render_stats *stats, void *stream) {
+  return render_noalloc(s, camera, desc, d_sum, d_sumsq, stats, stream);
+}
 
 int tpt_render(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum, float *d_sumsq,
                tpt_render_stats *stats, void *stream) {
