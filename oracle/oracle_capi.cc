@@ -37,14 +37,18 @@ struct philox_rng : oracle::rng {
 struct mt_rng : oracle::rng {
   std::mt19937 gen;  // default seed 5489, as a fresh row thread of the reference
   std::uniform_real_distribution<double> dist{0.0, 1.0};
+  bool rtl = false;  // the reference build's argument order (rng::args_right_to_left)
   double next() override { return dist(gen); }
+  bool args_right_to_left() const override { return rtl; }
 };
 
 template <class F>
 void parallel_for(uint64_t n, int nthreads, F &&fn) {
   if (nthreads < 1) nthreads = 1;
   std::atomic<uint64_t> next{0};
-  const uint64_t chunk = 1024;
+  // chunks of up to 1024 indices, at least 8 per thread (image rows are few)
+  const uint64_t chunk =
+      std::max<uint64_t>(1, std::min<uint64_t>(1024, n / (8 * static_cast<uint64_t>(nthreads))));
   auto worker = [&]() {
     for (;;) {
       const uint64_t b = next.fetch_add(chunk);
@@ -309,7 +313,8 @@ int oracle_make_rays(const oracle_scene *s, int kind, uint64_t n, uint64_t seed,
 // Renders spp samples per pixel into sum/sumsq (3·w·h floats each, row 0 =
 // bottom, accumulated with +=).  rng_mode 0: the Philox path streams the
 // device uses; 1: a fresh default-seeded mt19937 per row (the reference's
-// thread-per-row generator state, main.cpp:157-159 + src/utils.cc:28-32).
+// thread-per-row generator state, main.cpp:157-159 + src/utils.cc:28-32);
+// 2: as 1, constructor arguments drawn last first (rng::args_right_to_left).
 int oracle_render(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint64_t seed,
                   uint32_t sample_offset, float background, int rng_mode, int nthreads, float *sum,
                   float *sumsq, int media_scatter) {
@@ -326,6 +331,7 @@ int oracle_render(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, u
       const uint64_t px = j * w + i;
       for (uint32_t k = 0; k < spp; ++k) {
         oracle::rng *R = &mt;
+        mt.rtl = rng_mode == 2;
         if (rng_mode == 0) {
           ph.s.pixel = static_cast<uint32_t>(px);
           ph.s.sample = sample_offset + k;
@@ -468,3 +474,221 @@ int oracle_perlin_eval(const float *grad, const int32_t *perm, const float *pts,
 }
 
 }  // extern "C"
+
+// ---- single functions, for bit-exact comparison with the reference's ----
+// Same arguments as the ref_* functions of oracle/ref_harness.cc; `rtl` picks
+// the argument order of multi-draw constructors (rng::args_right_to_left).
+namespace {
+
+hitable *make_light(int kind, const float *p) {
+  if (kind == 0) return new sphere(vec3(p[0], p[1], p[2]), p[3], nullptr);
+  if (kind == 1) return new xz_rect(p[0], p[1], p[2], p[3], p[4], nullptr);
+  if (kind == 2) {
+    hitable **l = new hitable *[2];
+    l[0] = new xz_rect(-100, 100, -150, -50, 298, nullptr);
+    l[1] = new sphere(vec3(120, -50, 40), 120, nullptr);
+    return new hitable_list(l, 2);
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_sampler(int kind, uint64_t n, int rtl, float *out) {
+  mt_rng R;
+  R.rtl = rtl != 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const vec3 v = kind == 0 ? oracle::unit_sphere_sample(R) : oracle::unit_disk_sample(R);
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+  }
+  return 0;
+}
+
+int oracle_light_pdf(int kind, const float *p, const float *o, const float *d, uint64_t n, float *out) {
+  hitable *h = make_light(kind, p);
+  if (!h) return -1;
+  oracle::graph_info g;
+  oracle::index_graph(h, &g);
+  for (uint64_t i = 0; i < n; ++i)
+    out[i] = oracle::light_pdf_value(h, g, vec3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                     vec3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+  return 0;
+}
+
+int oracle_light_random(int kind, const float *p, const float *o, uint64_t n, int rtl, float *out) {
+  hitable *h = make_light(kind, p);
+  if (!h) return -1;
+  mt_rng R;
+  R.rtl = rtl != 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const vec3 v = oracle::light_sample(h, vec3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), R);
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+  }
+  return 0;
+}
+
+int oracle_texture_value(int kind, const float *c, const unsigned char *texels, int w, int h, const float *q,
+                         uint64_t n, float *out) {
+  texture *t = nullptr;
+  if (kind == 0)
+    t = new checker_texture(new constant_texture(vec3(c[0], c[1], c[2])), new constant_texture(vec3(c[3], c[4], c[5])));
+  if (kind == 1) t = new image_texture(const_cast<unsigned char *>(texels), w, h);
+  if (!t) return -1;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float *x = q + 5 * i;
+    const vec3 v = oracle::texture_value(t, x[0], x[1], vec3(x[2], x[3], x[4]));
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+  }
+  return 0;
+}
+
+int oracle_metal_scatter(const float *albedo, float fuzz, const float *q, uint64_t n, int rtl, float *dir,
+                         int32_t *ok) {
+  const metal m(vec3(albedo[0], albedo[1], albedo[2]), fuzz);
+  mt_rng R;
+  R.rtl = rtl != 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float *x = q + 13 * i;
+    const ray r(vec3(x[0], x[1], x[2]), vec3(x[3], x[4], x[5]), x[6]);
+    hit_record rec;
+    rec.t = 1.0f;
+    rec.point = vec3(x[7], x[8], x[9]);
+    rec.normal = vec3(x[10], x[11], x[12]);
+    ray out;
+    ok[i] = oracle::metal_scatter(&m, r, rec, R, &out) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) dir[3 * i + k] = out.direction()[k];
+  }
+  return 0;
+}
+
+// The sample loop of ref_pixel_samples with the restated color(): each
+// listed pixel from a fresh mt19937.
+int oracle_pixel_samples(const oracle_scene *s, const float *camp, uint32_t w, uint32_t h, const uint32_t *pix,
+                         uint64_t npix, uint32_t spp, uint32_t max_depth, int rtl, int nthreads, float *out) {
+  if (!s->spec.lights) return -1;
+  const camera_with_blur cam(vec3(camp[0], camp[1], camp[2]), vec3(camp[3], camp[4], camp[5]),
+                             vec3(camp[6], camp[7], camp[8]), camp[9], camp[10], camp[11], camp[12], camp[13],
+                             camp[14]);
+  oracle::render_opts o;
+  o.max_depth = static_cast<int>(max_depth);
+  parallel_for(npix, nthreads, [&](uint64_t k) {
+    mt_rng R;
+    R.rtl = rtl != 0;
+    const uint32_t i = pix[2 * k], j = pix[2 * k + 1];
+    for (uint32_t smp = 0; smp < spp; ++smp) {
+      const float u = static_cast<float>((static_cast<float>(i) + R.next()) / static_cast<float>(w));
+      const float v = static_cast<float>((static_cast<float>(j) + R.next()) / static_cast<float>(h));
+      const ray r = oracle::camera_ray(cam, u, v, R);
+      const vec3 c = oracle::color(r, s->spec.world, s->g, s->spec.lights, 0, o, R);
+      for (int ch = 0; ch < 3; ++ch) out[(k * spp + smp) * 3 + ch] = std::isnan(c[ch]) ? 0.0f : c[ch];
+    }
+  });
+  return 0;
+}
+
+// color() of explicit rays (6 x n planes, as oracle_trace) with the Philox
+// streams of the device (pixel = ray index, sample = `sample`), misses
+// returning the sky gradient times `background`.  out: 3 x n.
+int oracle_color_rays(const oracle_scene *s, const float *rays, uint64_t n, uint32_t max_depth, float background,
+                      uint64_t seed, uint32_t sample, int nthreads, float *out) {
+  oracle::render_opts o;
+  o.max_depth = static_cast<int>(max_depth);
+  o.background = background;
+  parallel_for(n, nthreads, [&](uint64_t i) {
+    philox_rng R;
+    R.s.k0 = static_cast<uint32_t>(seed);
+    R.s.k1 = static_cast<uint32_t>(seed >> 32);
+    R.s.pixel = static_cast<uint32_t>(i);
+    R.s.sample = sample;
+    R.s.set_bounce(0);
+    const ray r(vec3(rays[i], rays[n + i], rays[2 * n + i]), vec3(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]));
+    const vec3 c = oracle::color(r, s->spec.world, s->g, s->spec.lights, 0, o, R);
+    for (int ch = 0; ch < 3; ++ch) out[ch * n + i] = c[ch];
+  });
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- the guided estimator (oracle_guide.cc) ----
+extern "C" {
+
+// Network outputs (n x 40, before the bf16 rounding) for inputs in9 (n x 9).
+int oracle_guide_forward(const float *params, const float *in9, uint64_t n, float *out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    double o[40];
+    oracle::guide_forward(params, in9 + 9 * i, o);
+    for (int k = 0; k < 40; ++k) out[40 * i + k] = static_cast<float>(o[k]);
+  }
+  return 0;
+}
+
+// The decoded mixture of input in9: n directions sampled with draws u (n x 3)
+// and the mixture density at each (of the unit direction); -1 if the
+// network output is not finite.
+int oracle_guide_sample(const float *params, const float *in9, const float *u, uint64_t n, float *dirs, float *pdf) {
+  oracle::guide_mix m;
+  if (!oracle::guide_decode(params, in9, &m)) return -1;
+  for (uint64_t i = 0; i < n; ++i) {
+    const vec3 d = oracle::guide_mix_sample(m, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    for (int k = 0; k < 3; ++k) dirs[3 * i + k] = d[k];
+    pdf[i] = oracle::guide_mix_pdf(m, unit_vector(d));
+  }
+  return 0;
+}
+
+// Mixture density of input in9 at n unit directions.
+int oracle_guide_pdf(const float *params, const float *in9, const float *dirs, uint64_t n, float *pdf) {
+  oracle::guide_mix m;
+  if (!oracle::guide_decode(params, in9, &m)) return -1;
+  for (uint64_t i = 0; i < n; ++i) pdf[i] = oracle::guide_mix_pdf(m, vec3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+  return 0;
+}
+
+// oracle_render (Philox streams) with the guided estimator: the first
+// guide_verts lambertian vertices of every path sample from the proposal of
+// `params` (padded layout + mix logit), the scene box mapping hit points to
+// the network input.
+int oracle_render_guided(const oracle_scene *s, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
+                         uint64_t seed, uint32_t sample_offset, const float *params, int guide_verts, int nthreads,
+                         float *sum, float *sumsq) {
+  oracle::render_opts o;
+  o.max_depth = static_cast<int>(max_depth);
+  o.guide = params;
+  o.guide_verts = guide_verts;
+  for (int k = 0; k < 3; ++k) o.box_lo[k] = s->world_box.min_[k], o.box_hi[k] = s->world_box.max_[k];
+  float bg = 0.0f;
+  {
+    uint32_t a, b, c, d;
+    oracle_scene_settings(s, &a, &b, &c, &d, &bg);
+  }
+  o.background = bg;
+  parallel_for(h, nthreads, [&](uint64_t j) {
+    philox_rng ph;
+    ph.s.k0 = static_cast<uint32_t>(seed);
+    ph.s.k1 = static_cast<uint32_t>(seed >> 32);
+    for (uint32_t i = 0; i < w; ++i) {
+      const uint64_t px = j * w + i;
+      for (uint32_t k = 0; k < spp; ++k) {
+        ph.s.pixel = static_cast<uint32_t>(px);
+        ph.s.sample = sample_offset + k;
+        ph.s.set_bounce(0);
+        const float u = static_cast<float>((static_cast<float>(i) + ph.next()) / static_cast<float>(w));
+        const float v = static_cast<float>((static_cast<float>(j) + ph.next()) / static_cast<float>(h));
+        const ray r = oracle::camera_ray(*s->spec.cam, u, v, ph);
+        const vec3 c = oracle::color(r, s->spec.world, s->g, s->spec.lights, 0, o, ph);
+        for (int ch = 0; ch < 3; ++ch) {
+          const float x = std::isnan(c[ch]) ? 0.0f : c[ch];
+          sum[px * 3 + ch] += x;
+          if (sumsq) sumsq[px * 3 + ch] += x * x;
+        }
+      }
+    }
+  });
+  return 0;
+}
+
+}  // extern "C"
+

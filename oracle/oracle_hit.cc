@@ -281,7 +281,18 @@ bool hit(const hitable *h, const graph_info &g, const ray &r, float t_min, float
     if (hit_distance < distance_inside) {
       rec.t = rec1.t + hit_distance / r.direction().length();
       rec.point = r.point_at_parameter(rec.t);
-      const double n0 = R->medium(base, 1), n1 = R->medium(base, 2), n2 = R->medium(base, 3);
+      // vec3(drand_r(), drand_r(), drand_r()) (src/hitable.cc:120), in the
+      // source's argument order
+      double n0, n1, n2;
+      if (R->args_right_to_left()) {
+        n2 = R->medium(base, 3);
+        n1 = R->medium(base, 2);
+        n0 = R->medium(base, 1);
+      } else {
+        n0 = R->medium(base, 1);
+        n1 = R->medium(base, 2);
+        n2 = R->medium(base, 3);
+      }
       rec.normal = vec3(static_cast<float>(n0), static_cast<float>(n1), static_cast<float>(n2));
       rec.normal.normalize();
       rec.u = rec.v = 0.0f;  // not set by the reference

@@ -17,24 +17,47 @@ namespace {
 const float kFltMax = std::numeric_limits<float>::max();
 
 // ------------------------------------------------------------- samplers ---
-// src/utils.cc:13-19.  Draw order within one vector: x then y.
+// Two or three draws that the reference takes inside one constructor call,
+// in the order the random source says (rng::args_right_to_left).
+void draw2(rng &R, double *a, double *b) {
+  if (R.args_right_to_left()) {
+    *b = R.next();
+    *a = R.next();
+  } else {
+    *a = R.next();
+    *b = R.next();
+  }
+}
+
+void draw3(rng &R, double *a, double *b, double *c) {
+  if (R.args_right_to_left()) {
+    *c = R.next();
+    *b = R.next();
+    *a = R.next();
+  } else {
+    *a = R.next();
+    *b = R.next();
+    *c = R.next();
+  }
+}
+
+// src/utils.cc:13-19.
 vec3 in_unit_disk(rng &R) {
   vec3 p;
   do {
-    const double a = R.next();
-    const double b = R.next();
+    double a, b;
+    draw2(R, &a, &b);
     p = 2.0f * vec3(static_cast<float>(a), static_cast<float>(b), 0) - vec3(1, 1, 0);
   } while (dot(p, p) >= 1.0);
   return p;
 }
 
-// src/utils.cc:21-27.  Draw order x, y, z.
+// src/utils.cc:21-27.
 vec3 in_unit_sphere(rng &R) {
   vec3 p;
   do {
-    const double a = R.next();
-    const double b = R.next();
-    const double c = R.next();
+    double a, b, c;
+    draw3(R, &a, &b, &c);
     p = 2.0f * vec3(static_cast<float>(a), static_cast<float>(b), static_cast<float>(c)) - vec3(1, 1, 1);
   } while (p.length() >= 1.0);
   return p;
@@ -190,8 +213,8 @@ vec3 light_random(const hitable *h, const vec3 &o, rng &R) {
     return light_random(l->list_[idx], o, R);
   }
   if (auto *r = dynamic_cast<const xz_rect *>(h)) {
-    const double a = R.next();
-    const double b = R.next();
+    double a, b;
+    draw2(R, &a, &b);
     const vec3 pt(static_cast<float>(r->x0_ + a * (r->x1_ - r->x0_)), r->k_,
                   static_cast<float>(r->z0_ + b * (r->z1_ - r->z0_)));
     return pt - o;
@@ -219,6 +242,20 @@ vec3 sky(const ray &r, float scale) {
 
 }  // namespace
 
+vec3 unit_sphere_sample(rng &R) { return in_unit_sphere(R); }
+vec3 unit_disk_sample(rng &R) { return in_unit_disk(R); }
+float light_pdf_value(const hitable *lights, const graph_info &g, const vec3 &o, const vec3 &d) {
+  return light_pdf(lights, g, o, d);
+}
+vec3 light_sample(const hitable *lights, const vec3 &o, rng &R) { return light_random(lights, o, R); }
+
+// src/material.cc:88-98.
+bool metal_scatter(const metal *m, const ray &r, const hit_record &rec, rng &R, ray *out) {
+  const vec3 refl = reflect(unit_vector(r.direction()), rec.normal);
+  *out = ray(rec.point, refl + m->fuzz_ * in_unit_sphere(R), r.time());
+  return dot(out->direction(), rec.normal) > 0;
+}
+
 float perlin_noise_at(const vec3 &p) { return perlin(p); }
 float perlin_turb(const vec3 &p, int depth) { return turb(p, depth); }
 vec3 texture_value(const texture *t, float u, float v, const vec3 &p) { return tex_value(t, u, v, p); }
@@ -234,11 +271,7 @@ ray camera_ray(const camera_with_blur &cam, float s, float t, rng &R) {
 bool scatter_once(const ray &r, const hit_record &rec, const graph_info &g, const hitable *lights, rng &R,
                   ray *out) {
   const material *m = rec.mat_ptr;
-  if (auto *me = dynamic_cast<const metal *>(m)) {
-    const vec3 refl = reflect(unit_vector(r.direction()), rec.normal);
-    *out = ray(rec.point, refl + me->fuzz_ * in_unit_sphere(R), r.time());
-    return dot(out->direction(), rec.normal) > 0;
-  }
+  if (auto *me = dynamic_cast<const metal *>(m)) return metal_scatter(me, r, rec, R, out);
   if (auto *de = dynamic_cast<const dielectric *>(m)) {
     const vec3 d = r.direction();
     const float ior = de->ref_idx_;
@@ -262,7 +295,7 @@ bool scatter_once(const ray &r, const hit_record &rec, const graph_info &g, cons
 }
 
 vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitable *lights, int depth,
-           const render_opts &o, rng &R) {
+           const render_opts &o, rng &R, int ndiff) {
   R.bounce(static_cast<uint32_t>(depth + 1));
   hit_record rec;
   uint32_t leaf = 0;
@@ -278,11 +311,9 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
   }
   if (depth >= o.max_depth) return emitted;
   if (auto *me = dynamic_cast<const metal *>(m)) {
-    // src/material.cc:88-98.
-    const vec3 refl = reflect(unit_vector(r.direction()), rec.normal);
-    const ray scattered(rec.point, refl + me->fuzz_ * in_unit_sphere(R), r.time());
-    if (!(dot(scattered.direction(), rec.normal) > 0)) return emitted;
-    return me->albedo_ * color(scattered, world, g, lights, depth + 1, o, R);
+    ray scattered;
+    if (!metal_scatter(me, r, rec, R, &scattered)) return emitted;
+    return me->albedo_ * color(scattered, world, g, lights, depth + 1, o, R, ndiff);
   }
   if (auto *de = dynamic_cast<const dielectric *>(m)) {
     // src/material.cc:19-70.
@@ -303,7 +334,7 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
     }
     reflect_prob = refract(d, outward, ni_over_nt, refracted) ? schlick(cosine, ior) : 1.0f;
     const ray scattered(rec.point, R.next() < reflect_prob ? reflected : refracted, r.time());
-    return vec3(1, 1, 1) * color(scattered, world, g, lights, depth + 1, o, R);
+    return vec3(1, 1, 1) * color(scattered, world, g, lights, depth + 1, o, R, ndiff);
   }
   if (auto *iso = dynamic_cast<const isotropic *>(m)) {
     // The reference's isotropic::scatter does not override material::scatter
@@ -311,13 +342,54 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
     // mode follows src/material.cc:72-77 instead.
     if (!o.media_scatter) return emitted;
     const ray scattered(rec.point, in_unit_sphere(R), r.time());
-    return tex_value(iso->albedo_, rec.u, rec.v, rec.point) * color(scattered, world, g, lights, depth + 1, o, R);
+    return tex_value(iso->albedo_, rec.u, rec.v, rec.point) *
+           color(scattered, world, g, lights, depth + 1, o, R, ndiff);
   }
   auto *lam = dynamic_cast<const lambertian *>(m);
   if (!lam) return emitted;
   // src/material.cc:3-17 + src/utils.cc:73-81.
   const vec3 atten = tex_value(lam->albedo_, rec.u, rec.v, rec.point);
   const onb uvw(rec.normal);
+  if (o.guide && ndiff < o.guide_verts) {
+    // Guided vertex (no reference counterpart): directions from
+    // m = a g + (1 - a) q, q = 0.5 p_light + 0.5 cos/pi (the reference
+    // sampler's true density), weight atten (cos/pi) q / (p_ref m) with p_ref
+    // the reference's mixture value; the device's draw order (k_guide_shade).
+    float in9[9];
+    for (int k = 0; k < 3; ++k)
+      in9[k] = (rec.point[k] - o.box_lo[k]) / std::fmax(o.box_hi[k] - o.box_lo[k], 1e-20f);
+    const vec3 wo = -unit_vector(r.direction());
+    for (int k = 0; k < 3; ++k) in9[3 + k] = rec.normal[k], in9[6 + k] = wo[k];
+    guide_mix mix;
+    const bool finite = guide_decode(o.guide, in9, &mix);
+    const float alpha = finite ? guide_alpha(o.guide) : 0.0f;  // a non-finite output: the reference sampler
+    vec3 nd;
+    if (R.next() < alpha) {
+      const float u0 = static_cast<float>(R.next()), u1 = static_cast<float>(R.next()),
+                  u2 = static_cast<float>(R.next());
+      nd = guide_mix_sample(mix, u0, u1, u2);
+    } else if (lights && R.next() < 0.5) {
+      nd = light_random(lights, rec.point, R);
+    } else {
+      const vec3 h = on_hemisphere(R);
+      nd = uvw.local(h.x(), h.y(), h.z());
+    }
+    const vec3 ud = unit_vector(nd);
+    const float kPi = static_cast<float>(M_PI);
+    const float gv = alpha > 0.0f ? guide_mix_pdf(mix, ud) : 0.0f;
+    const float pl = lights ? light_pdf(lights, g, rec.point, nd) : 0.0f;
+    float pc = dot(ud, uvw.w);
+    pc = pc > 0.0f ? pc : 0.0f;
+    const float p_ref = lights ? 0.5f * pl + 0.5f * pc : pc;
+    const float q = lights ? 0.5f * pl + 0.5f * (pc / kPi) : pc / kPi;
+    const float m = alpha > 0.0f ? alpha * gv + (1.0f - alpha) * q : q;
+    float spdf = dot(rec.normal, ud);
+    spdf = spdf < 0.0f ? 0.0f : spdf / kPi;
+    // below the surface and off the light the integrand is zero: the path ends
+    if (!(spdf > 0.0f && q > 0.0f)) return emitted;
+    const vec3 w = atten * (spdf * q) / (p_ref * m);
+    return emitted + w * color(ray(rec.point, nd, r.time()), world, g, lights, depth + 1, o, R, ndiff + 1);
+  }
   vec3 dir;
   float pdf;
   if (lights) {
@@ -344,7 +416,7 @@ vec3 color(const ray &r, const hitable *world, const graph_info &g, const hitabl
   const ray scattered(rec.point, dir, r.time());
   float spdf = dot(rec.normal, unit_vector(scattered.direction()));
   spdf = spdf < 0 ? 0.0f : static_cast<float>(spdf / M_PI);
-  return emitted + atten * spdf * color(scattered, world, g, lights, depth + 1, o, R) / pdf;
+  return emitted + atten * spdf * color(scattered, world, g, lights, depth + 1, o, R, ndiff + 1) / pdf;
 }
 
 }  // namespace oracle

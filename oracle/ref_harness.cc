@@ -401,3 +401,149 @@ int ref_perlin_eval(const float *grad, const int32_t *perm, const float *pts, ui
 }
 
 }  // extern "C"
+
+// ---- single functions of the reference, for bit-exact pins of the oracle ----
+// Every function that draws runs on a fresh thread, so drand_r starts from its
+// default state (a std::mt19937 seeded 5489), as on each row thread of the
+// reference's driver.
+namespace {
+
+// Light shapes: kind 0 sphere (cx, cy, cz, r); kind 1 xz_rect (x0, x1, z0, z1,
+// k); kind 2 the reference driver's light list (main.cpp:99-106).
+hitable *make_light(int kind, const float *p) {
+  if (kind == 0) return new sphere(vec3(p[0], p[1], p[2]), p[3], nullptr);
+  if (kind == 1) return new xz_rect(p[0], p[1], p[2], p[3], p[4], nullptr);
+  if (kind == 2) {
+    hitable **l = new hitable *[2];
+    l[0] = new xz_rect(-100, 100, -150, -50, 298, nullptr);
+    l[1] = new sphere(vec3(120, -50, 40), 120, nullptr);
+    return new hitable_list(l, 2);
+  }
+  return nullptr;
+}
+
+template <class F>
+void on_fresh_thread(F &&fn) {
+  std::thread t(fn);
+  t.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+// random_in_unit_sphere() (kind 0) or random_in_unit_disk() (kind 1), n
+// successive samples, 3 floats each.
+int ref_sampler(int kind, uint64_t n, float *out) {
+  on_fresh_thread([&] {
+    for (uint64_t i = 0; i < n; ++i) {
+      const vec3 v = kind == 0 ? random_in_unit_sphere() : random_in_unit_disk();
+      for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+    }
+  });
+  return 0;
+}
+
+// hitable::pdf_value(o[i], d[i]) of a light shape (make_light).
+int ref_light_pdf(int kind, const float *p, const float *o, const float *d, uint64_t n, float *out) {
+  hitable *h = make_light(kind, p);
+  if (!h) return -1;
+  for (uint64_t i = 0; i < n; ++i)
+    out[i] = h->pdf_value(vec3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), vec3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+  return 0;
+}
+
+// hitable::random(o[i]) of a light shape, successive draws of one generator.
+int ref_light_random(int kind, const float *p, const float *o, uint64_t n, float *out) {
+  hitable *h = make_light(kind, p);
+  if (!h) return -1;
+  on_fresh_thread([&] {
+    for (uint64_t i = 0; i < n; ++i) {
+      const vec3 v = h->random(vec3(o[3 * i], o[3 * i + 1], o[3 * i + 2]));
+      for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+    }
+  });
+  return 0;
+}
+
+// texture::value(u, v, p) for queries q[i] = (u, v, px, py, pz): kind 0
+// checker_texture(constant_texture(c[0..2]), constant_texture(c[3..5])) (odd,
+// even); kind 1 image_texture(texels, w, h).
+int ref_texture_value(int kind, const float *c, const unsigned char *texels, int w, int h, const float *q, uint64_t n,
+                      float *out) {
+  texture *t = nullptr;
+  if (kind == 0)
+    t = new checker_texture(new constant_texture(vec3(c[0], c[1], c[2])), new constant_texture(vec3(c[3], c[4], c[5])));
+  if (kind == 1) t = new image_texture(const_cast<unsigned char *>(texels), w, h);
+  if (!t) return -1;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float *x = q + 5 * i;
+    const vec3 v = t->value(x[0], x[1], vec3(x[2], x[3], x[4]));
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = v[k];
+  }
+  return 0;
+}
+
+// metal(albedo, fuzz)::scatter for queries q[i] = (ray o, ray d, time, rec
+// point, rec normal): the scattered direction and the return value,
+// successive draws of one generator.
+int ref_metal_scatter(const float *albedo, float fuzz, const float *q, uint64_t n, float *dir, int32_t *ok) {
+  metal m(vec3(albedo[0], albedo[1], albedo[2]), fuzz);
+  on_fresh_thread([&] {
+    for (uint64_t i = 0; i < n; ++i) {
+      const float *x = q + 13 * i;
+      const ray r(vec3(x[0], x[1], x[2]), vec3(x[3], x[4], x[5]), x[6]);
+      hit_record rec;
+      rec.t = 1.0f;
+      rec.point = vec3(x[7], x[8], x[9]);
+      rec.normal = vec3(x[10], x[11], x[12]);
+      rec.mat_ptr = &m;
+      scatter_record sr;
+      ok[i] = m.scatter(r, rec, sr) ? 1 : 0;
+      for (int k = 0; k < 3; ++k) dir[3 * i + k] = sr.specular_ray.direction()[k];
+    }
+  });
+  return 0;
+}
+
+// The reference driver's sample loop (main.cpp:115-126) for listed pixels
+// pix[2k] = i, pix[2k + 1] = j: u, v, get_ray, de_nan(color()), spp samples
+// each, every pixel on a fresh thread (a fresh generator).  out: npix * spp
+// * 3 floats.  Camera from params as in ref_camera_fields.
+int ref_pixel_samples(void *sp, const float *camp, uint32_t w, uint32_t h, const uint32_t *pix, uint64_t npix,
+                      uint32_t spp, uint32_t max_depth, int nthreads, float *out) {
+  auto *s = static_cast<ref_scene_t *>(sp);
+  if (!s->lights) return -1;
+  camera cam(vec3(camp[0], camp[1], camp[2]), vec3(camp[3], camp[4], camp[5]), vec3(camp[6], camp[7], camp[8]),
+             camp[9], camp[10], camp[11], camp[12], camp[13], camp[14]);
+  auto pixel = [&](uint64_t k) {
+    const uint32_t i = pix[2 * k], j = pix[2 * k + 1];
+    for (uint32_t smp = 0; smp < spp; ++smp) {
+      const float u = ((float)i + drand_r()) / (float)w;
+      const float v = ((float)j + drand_r()) / (float)h;
+      ray r = cam.get_ray(u, v);
+      const vec3 c = de_nan(color(r, s->render_world, s->lights, 0, static_cast<int>(max_depth)));
+      for (int ch = 0; ch < 3; ++ch) out[(k * spp + smp) * 3 + ch] = c[ch];
+    }
+  };
+  if (nthreads < 1) nthreads = 1;
+  for (uint64_t b = 0; b < npix; b += static_cast<uint64_t>(nthreads)) {
+    std::vector<std::thread> ts;
+    for (uint64_t k = b; k < std::min<uint64_t>(npix, b + nthreads); ++k) ts.emplace_back(pixel, k);
+    for (auto &t : ts) t.join();
+  }
+  return 0;
+}
+
+// Replaces the reference's static Perlin tables (its constructor shuffles
+// them from the clock), so that both sides evaluate the same noise.
+void ref_perlin_install(const float *grad, const int32_t *perm) {
+  for (int i = 0; i < 256; ++i) {
+    perlin_noise::random_vec3_[i] = vec3(grad[3 * i], grad[3 * i + 1], grad[3 * i + 2]);
+    perlin_noise::permute_x_[i] = perm[i];
+    perlin_noise::permute_y_[i] = perm[256 + i];
+    perlin_noise::permute_z_[i] = perm[512 + i];
+  }
+}
+
+}  // extern "C"

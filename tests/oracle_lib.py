@@ -18,6 +18,7 @@ REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtpt_ref.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 
 RAYS_PRIMARY, RAYS_SECONDARY, RAYS_UNIFORM, RAYS_ADVERSARIAL = 0, 1, 2, 3
 
@@ -38,6 +39,49 @@ def _perlin_eval(fn, grad, perm, pts, scale, octaves):
         args.append(C.c_int(octaves))
     fn(*args, noise.ctypes.data, turb.ctypes.data, tex.ctypes.data)
     return noise, turb, tex
+
+
+def _light_pdf(fn, kind, params, o, d):
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros(o.shape[0], np.float32)
+    assert fn(kind, np.asarray(params, np.float32), o.reshape(-1), d.reshape(-1), o.shape[0], out) == 0
+    return out
+
+
+def _light_random(fn, kind, params, o, rtl=None):
+    o = np.ascontiguousarray(o, np.float32)
+    out = np.zeros(3 * o.shape[0], np.float32)
+    args = [kind, np.asarray(params, np.float32), o.reshape(-1), o.shape[0]]
+    if rtl is not None:
+        args.append(rtl)
+    assert fn(*args, out) == 0
+    return out.reshape(-1, 3)
+
+
+def _texture_value(fn, kind, colors, texels, q):
+    q = np.ascontiguousarray(q, np.float32)
+    out = np.zeros(3 * q.shape[0], np.float32)
+    c = np.zeros(6, np.float32) if colors is None else np.asarray(colors, np.float32)
+    if texels is None:
+        ptr, w, h = None, 0, 0
+    else:
+        texels = np.ascontiguousarray(texels, np.uint8)
+        ptr, w, h = texels.ctypes.data, texels.shape[1], texels.shape[0]
+    assert fn(kind, c, ptr, w, h, q.reshape(-1), q.shape[0], out) == 0
+    return out.reshape(-1, 3)
+
+
+def _metal_scatter(fn, albedo, fuzz, q, rtl=None):
+    q = np.ascontiguousarray(q, np.float32)
+    n = q.shape[0]
+    d = np.zeros(3 * n, np.float32)
+    ok = np.zeros(n, np.int32)
+    args = [np.asarray(albedo, np.float32), fuzz, q.reshape(-1), n]
+    if rtl is not None:
+        args.append(rtl)
+    fn(*args, d, ok)
+    return d.reshape(n, 3), ok
 
 
 class Oracle:
@@ -65,6 +109,20 @@ class Oracle:
         L.oracle_prim_hit.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_float, C.c_float, _f32p, _f32p]
         L.oracle_camera_from_params.argtypes = [_f32p, _f32p]
         L.oracle_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
+        L.oracle_sampler.argtypes = [C.c_int, C.c_uint64, C.c_int, _f32p]
+        L.oracle_light_pdf.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_uint64, _f32p]
+        L.oracle_light_random.argtypes = [C.c_int, _f32p, _f32p, C.c_uint64, C.c_int, _f32p]
+        L.oracle_texture_value.argtypes = [C.c_int, _f32p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_uint64, _f32p]
+        L.oracle_metal_scatter.argtypes = [_f32p, C.c_float, _f32p, C.c_uint64, C.c_int, _f32p, _i32p]
+        L.oracle_pixel_samples.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint64,
+                                           C.c_uint32, C.c_uint32, C.c_int, C.c_int, _f32p]
+        L.oracle_color_rays.argtypes = [C.c_void_p, _f32p, C.c_uint64, C.c_uint32, C.c_float, C.c_uint64,
+                                        C.c_uint32, C.c_int, _f32p]
+        L.oracle_guide_forward.argtypes = [_f32p, _f32p, C.c_uint64, _f32p]
+        L.oracle_guide_sample.argtypes = [_f32p, _f32p, _f32p, C.c_uint64, _f32p, _f32p]
+        L.oracle_guide_pdf.argtypes = [_f32p, _f32p, _f32p, C.c_uint64, _f32p]
+        L.oracle_render_guided.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                           C.c_uint32, _f32p, C.c_int, C.c_int, _f32p, C.c_void_p]
 
     def perlin_tables(self):
         """The scene API's current Perlin tables: grad (256, 3), perm (3, 256)."""
@@ -101,6 +159,49 @@ class Oracle:
 
     def scene(self, name, seed=5489):
         return OracleScene(self, name, seed)
+
+    # -- the guided proposal (no reference counterpart) --
+    def guide_forward(self, params, in9):
+        in9 = np.ascontiguousarray(in9, np.float32)
+        out = np.zeros(40 * in9.shape[0], np.float32)
+        self.L.oracle_guide_forward(np.ascontiguousarray(params, np.float32), in9.reshape(-1), in9.shape[0], out)
+        return out.reshape(-1, 40)
+
+    def guide_sample(self, params, in9, u):
+        """Directions sampled from the decoded mixture of one input with draws
+        u (n, 3), and the mixture density at each."""
+        u = np.ascontiguousarray(u, np.float32)
+        d = np.zeros(3 * u.shape[0], np.float32)
+        pdf = np.zeros(u.shape[0], np.float32)
+        rc = self.L.oracle_guide_sample(np.ascontiguousarray(params, np.float32), np.asarray(in9, np.float32),
+                                        u.reshape(-1), u.shape[0], d, pdf)
+        assert rc == 0
+        return d.reshape(-1, 3), pdf
+
+    def guide_pdf(self, params, in9, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        pdf = np.zeros(dirs.shape[0], np.float32)
+        assert self.L.oracle_guide_pdf(np.ascontiguousarray(params, np.float32), np.asarray(in9, np.float32),
+                                       dirs.reshape(-1), dirs.shape[0], pdf) == 0
+        return pdf
+
+    # -- single functions (same arguments as Reference's) --
+    def sampler(self, kind, n, rtl):
+        out = np.zeros(3 * n, np.float32)
+        self.L.oracle_sampler(kind, n, rtl, out)
+        return out.reshape(n, 3)
+
+    def light_pdf(self, kind, params, o, d):
+        return _light_pdf(self.L.oracle_light_pdf, kind, params, o, d)
+
+    def light_random(self, kind, params, o, rtl):
+        return _light_random(self.L.oracle_light_random, kind, params, o, rtl)
+
+    def texture_value(self, kind, colors, texels, q):
+        return _texture_value(self.L.oracle_texture_value, kind, colors, texels, q)
+
+    def metal_scatter(self, albedo, fuzz, q, rtl):
+        return _metal_scatter(self.L.oracle_metal_scatter, albedo, fuzz, q, rtl)
 
 
 class OracleScene:
@@ -155,6 +256,37 @@ class OracleScene:
                                 nthreads or _nthreads(), None if time is None else time.ctypes.data)
         return (leaf, t, rec, mat) if record else (leaf, t)
 
+    def pixel_samples(self, cam_params, width, height, pix, spp, max_depth, rtl, nthreads=None):
+        """Per-sample de_nan'ed colors (npix, spp, 3) of the listed pixels, each
+        pixel from a fresh mt19937 (as Reference.pixel_samples)."""
+        pix = np.ascontiguousarray(pix, np.uint32)
+        out = np.zeros(pix.shape[0] * spp * 3, np.float32)
+        rc = self.lib.L.oracle_pixel_samples(self.h, np.asarray(cam_params, np.float32), width, height,
+                                             pix.reshape(-1), pix.shape[0], spp, max_depth, rtl,
+                                             nthreads or _nthreads(), out)
+        assert rc == 0, "pixel samples need a light list"
+        return out.reshape(pix.shape[0], spp, 3)
+
+    def color_rays(self, rays, max_depth=50, background=1.0, seed=1, sample=0, nthreads=None):
+        """color() of explicit rays (6, n) with the device's Philox streams; (3, n)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[1]
+        out = np.zeros(3 * n, np.float32)
+        self.lib.L.oracle_color_rays(self.h, rays.reshape(-1), n, max_depth, background, seed, sample,
+                                     nthreads or _nthreads(), out)
+        return out.reshape(3, n)
+
+    def render_guided(self, width, height, spp, params, guide_verts=2, max_depth=None, seed=1, sample_offset=0,
+                      nthreads=None):
+        """The guided estimator on the device's Philox streams (sums, sumsq)."""
+        s = np.zeros(width * height * 3, np.float32)
+        sq = np.zeros(width * height * 3, np.float32)
+        self.lib.L.oracle_render_guided(self.h, width, height, spp,
+                                        max_depth if max_depth is not None else self.max_depth, seed, sample_offset,
+                                        np.ascontiguousarray(params, np.float32), guide_verts,
+                                        nthreads or _nthreads(), s, sq.ctypes.data)
+        return s.reshape(height, width, 3), sq.reshape(height, width, 3)
+
     def render(self, width, height, spp, max_depth=None, seed=1, sample_offset=0, background=None,
                rng_mode=0, nthreads=None, media_scatter=False):
         s = np.zeros(width * height * 3, np.float32)
@@ -183,6 +315,14 @@ class Reference:
         L.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, C.c_int]
         L.ref_render.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p,
                                  C.c_void_p, C.c_int]
+        L.ref_sampler.argtypes = [C.c_int, C.c_uint64, _f32p]
+        L.ref_light_pdf.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_uint64, _f32p]
+        L.ref_light_random.argtypes = [C.c_int, _f32p, _f32p, C.c_uint64, _f32p]
+        L.ref_texture_value.argtypes = [C.c_int, _f32p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_uint64, _f32p]
+        L.ref_metal_scatter.argtypes = [_f32p, C.c_float, _f32p, C.c_uint64, _f32p, _i32p]
+        L.ref_pixel_samples.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint64, C.c_uint32,
+                                        C.c_uint32, C.c_int, _f32p]
+        L.ref_perlin_install.argtypes = [C.c_void_p, C.c_void_p]
 
     def scene(self, name):
         h = C.c_void_p()
@@ -236,3 +376,35 @@ class Reference:
         if with_sumsq:
             return s.reshape(height, width, 3), sq.reshape(height, width, 3)
         return s.reshape(height, width, 3)
+
+    # -- single functions of the reference (fresh generator where they draw) --
+    def sampler(self, kind, n):
+        out = np.zeros(3 * n, np.float32)
+        self.L.ref_sampler(kind, n, out)
+        return out.reshape(n, 3)
+
+    def light_pdf(self, kind, params, o, d):
+        return _light_pdf(self.L.ref_light_pdf, kind, params, o, d)
+
+    def light_random(self, kind, params, o):
+        return _light_random(self.L.ref_light_random, kind, params, o)
+
+    def texture_value(self, kind, colors, texels, q):
+        return _texture_value(self.L.ref_texture_value, kind, colors, texels, q)
+
+    def metal_scatter(self, albedo, fuzz, q):
+        return _metal_scatter(self.L.ref_metal_scatter, albedo, fuzz, q)
+
+    def pixel_samples(self, h, cam_params, width, height, pix, spp, max_depth, nthreads=None):
+        pix = np.ascontiguousarray(pix, np.uint32)
+        out = np.zeros(pix.shape[0] * spp * 3, np.float32)
+        rc = self.L.ref_pixel_samples(h, np.asarray(cam_params, np.float32), width, height, pix.reshape(-1),
+                                      pix.shape[0], spp, max_depth, nthreads or _nthreads(), out)
+        assert rc == 0, "pixel samples need a light list"
+        return out.reshape(pix.shape[0], spp, 3)
+
+    def perlin_install(self, grad, perm):
+        grad = np.ascontiguousarray(grad, np.float32)
+        perm = np.ascontiguousarray(perm, np.int32)
+        self.L.ref_perlin_install(grad.ctypes.data, perm.ctypes.data)
+

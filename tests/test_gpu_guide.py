@@ -6,8 +6,10 @@
   oracle's backward at the same bf16 rounding points.
 - tpt_guide_train: the first Adam step is the textbook update of that gradient,
   and training on learnable data drives the loss toward the data's entropy.
-- tpt_render_guided: the MIS estimator keeps the plain estimator's expectation
-  (untrained and trained networks), and the recorded training rows are sound.
+- tpt_render_guided: on the same Philox streams it equals the oracle's guided
+  color() (oracle/oracle_guide.cc) to rounding; the MIS estimator keeps the
+  plain estimator's expectation (untrained and trained networks, and at the
+  full 1200x1200 Cornell box); the recorded training rows are sound.
 """
 import os
 import sys
@@ -44,6 +46,35 @@ def guide(tpt, device):
     return tpt.Guide(seed=1)
 
 
+def midpoint_distance(v):
+    """|v - m| for the midpoint m between the two bf16 values around each
+    float32 value v (where rounding to bf16 changes direction)."""
+    v32 = np.ascontiguousarray(v, np.float32)
+    mid = ((v32.view(np.uint32) & np.uint32(0xFFFF0000)) | np.uint32(0x8000)).view(np.float32)
+    return np.abs(v32.astype(np.float64) - mid.astype(np.float64))
+
+
+def edge_rows(p, in9, act):
+    """Rows with a bf16 operand closer to a rounding midpoint than the device's
+    float32 result can be from the oracle's: a frequency feature of band b
+    (sinpi/cospi, then b float angle doublings: within 2^b * 5e-7), a hidden
+    activation (a float32 sum of K products: within K * 2^-24 times the sum of
+    |terms|).  Only such a row may see an operand rounded one bf16 ulp the
+    other way."""
+    f = G.features(in9)
+    band = np.zeros(48)
+    band[:36] = (np.arange(36) % 12) // 2
+    margin_f = np.where(np.arange(48) < 36, 2.0 ** band * 5e-7, 0.0)
+    edge = (midpoint_distance(f) <= margin_f).any(1)
+    w1, w2, w3, w4, b1, b2, b3, b4, _ = G.unpack(p)
+    xin = G.bf16(f.astype(np.float32))
+    for w, b, h, k in ((w1, b1, act["h1"], 48), (w2, b2, act["h2"], 64), (w3, b3, act["h3"], 64)):
+        scale = np.abs(xin) @ np.abs(G.bf16(w)).T + np.abs(b)
+        edge |= ((midpoint_distance(h) <= k * 2.0 ** -24 * scale) & (h > 0)).any(1)
+        xin = G.bf16(h)
+    return edge
+
+
 @pytest.mark.parametrize("n", [1, 65, 1000, 200000])
 def test_infer_matches_oracle(guide, device, n):
     p = make_params()
@@ -51,15 +82,18 @@ def test_infer_matches_oracle(guide, device, n):
     assert np.array_equal(guide.get_params(), p)
     rows = G.random_rows(n, seed=11 + n)
     out = guide.infer(to_dev(rows[:, :9], device)).cpu().numpy()
-    ref, _ = G.forward(p, rows[:, :9])
+    ref, act = G.forward(p, rows[:, :9])
     err = np.abs(out - ref) / (1 + np.abs(ref))
-    # Rows agree to FP32 accumulation rounding, except where a feature or an
-    # activation sits on a bf16 rounding boundary and the device's last bit
-    # (sinf, accumulation order) rounds it the other way: one bf16 ulp of one
-    # operand, which moves the outputs by well under 0.05.
-    off = (err > 1e-4).any(axis=1)
-    print(f"n={n}: max relative error {err.max():.3g}; rows off by a bf16 flip {off.mean():.4f}")
-    assert off.mean() <= 0.01
+    # Rows agree to FP32 accumulation rounding, except where an operand sits
+    # at a bf16 rounding midpoint and the device's float32 value rounds it one
+    # bf16 ulp the other way: every row that differs more must have such an
+    # operand (a difference with no midpoint to explain it is a bug).
+    off = (err > 1e-5).any(axis=1)
+    unexplained = off & ~edge_rows(p, rows[:, :9], act)
+    print(f"n={n}: rows off by more than 1e-5: {off.mean():.4f} (max error {err.max():.3g}), "
+          f"without an operand at a bf16 midpoint: {unexplained.sum()}")
+    assert not unexplained.any()
+    assert off.mean() <= 0.01  # measured 0.0022-0.0040
     assert err.max() < 0.05
 
 
@@ -83,7 +117,7 @@ def test_gradient_matches_oracle(guide, device, n):
         a, b = g_dev[lo:hi], g_ref[lo:hi]
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         print(f"n={n} {name}: relative error {rel:.2e}")
-        assert rel < 1e-2, (name, rel)
+        assert rel < 1e-3, (name, rel)  # measured at most 5.0e-4 (W1, n = 50000)
     assert np.all(g_dev[~G.live_mask()] == 0)
     assert abs(g_dev[G.NPARAM] - g_ref[G.NPARAM]) <= 1e-3 * max(1e-3, abs(g_ref[G.NPARAM]))
 
@@ -237,3 +271,62 @@ def test_trained_guide_lowers_the_error(tpt, device):
     print(f"squared display error at {spp} spp: plain {e_plain:.3g}, guided {e_guided:.3g} "
           f"({e_plain / e_guided:.2f}x lower)")
     assert e_guided * 1.3 < e_plain
+
+
+GUIDED_SAME_STREAM = [("cornell555", 40, 16), ("random_spheres", 40, 8), ("ref_cornell_box", 32, 8)]
+
+
+@pytest.mark.parametrize("name,w,spp", GUIDED_SAME_STREAM)
+@pytest.mark.parametrize("verts", [1, 2])
+def test_guided_same_streams_match_oracle(tpt, oracle, device, name, w, spp, verts):
+    """tpt_render_guided against the oracle's guided color() on the same
+    Philox streams and the same parameters: pixels equal to rounding, except
+    where an fp32 last bit (the device's exp/log/sincos in the mixture, a
+    network output on a bf16 rounding midpoint) moves a sampled direction.
+    Scenes: a rect light (cornell555), no light list (the cosine-only q), and
+    the reference driver's rect + sphere light list with metal (ref_cornell_box)."""
+    p = make_params()
+    g = tpt.Guide(seed=1)
+    g.set_params(p)
+    sc = tpt.Scene.named(name)
+    s_g, q_g, st = sc.render(width=w, height=w, spp=spp, seed=17, camera=sc.host.camera, guide=g,
+                             guide_vertices=verts)
+    s_g, q_g = s_g.cpu().numpy(), q_g.cpu().numpy()
+    s_c, q_c = oracle.scene(name).render_guided(w, w, spp, p, guide_verts=verts, seed=17)
+    assert st["paths"] == w * w * spp
+    close = (np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)).all(axis=2)
+    frac = close.mean()
+    z = (s_g - s_c) / spp / np.sqrt((np.maximum(q_g / spp - (s_g / spp) ** 2, 0) +
+                                     np.maximum(q_c / spp - (s_c / spp) ** 2, 0)) / spp + 1e-12)
+    print(f"{name} verts={verts}: {frac:.4f} of pixels equal to rounding; |z|>4 in {np.mean(np.abs(z) > 4):.4f}; "
+          f"means {s_g.mean():.5f} {s_c.mean():.5f}")
+    assert frac >= 0.997  # measured 0.9988-1.0 (at most 2 pixels of 1600)
+    assert np.mean(np.abs(z) > 4) < 0.005
+
+
+def test_guided_keeps_the_expectation_at_full_size(tpt, device):
+    """The benchmark frame, Cornell 1200x1200: a guide trained by the online
+    schedule (3 passes of 4 spp recording every 8th path, 64 Adam steps each),
+    then 64 guided and 64 plain samples per pixel on independent streams:
+    image means within 4 standard errors, |z| > 3 in under 1.5% of the pixel
+    channels."""
+    sc = tpt.Scene.named("cornell555")
+    w, spp = 1200, 64
+    cam = sc.host.camera
+    g = tpt.Guide(seed=2)
+    r = tpt.ProgressiveRender(sc, guide=g, seed=71, record_stride=8, iters=64, batch=1 << 18, lr=5e-3)
+    for _ in range(3):
+        r.add(4, train=True)
+    s_g, q_g, st = sc.render(width=w, height=w, spp=spp, seed=72, camera=cam, guide=g)
+    s_p, q_p, _ = sc.render(width=w, height=w, spp=spp, seed=73, camera=cam)
+    s_g, q_g, s_p, q_p = (x.cpu().numpy().astype(np.float64) for x in (s_g, q_g, s_p, q_p))
+    assert st["paths"] == w * w * spp
+    frac, _ = zfrac(s_g, q_g, s_p, q_p, spp)
+    n = w * w * 3 * spp
+    se = np.sqrt(q_g.mean() / spp / n + q_p.mean() / spp / n)
+    dm = (s_g.mean() - s_p.mean()) / spp
+    alpha = 1 / (1 + np.exp(-g.get_params()[G.NPARAM]))
+    print(f"1200^2: alpha={alpha:.3f}; |z|>3 in {frac:.4f}; image mean diff {dm:.6f} ({dm / se:.2f} SE)")
+    assert abs(dm) <= 4 * se
+    assert frac < 0.015
+

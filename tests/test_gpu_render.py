@@ -31,20 +31,58 @@ def zscores(s1, q1, n1, s2, q2, n2):
     return (m1 - m2) / np.sqrt(v1 + v2 + 1e-12)
 
 
-@pytest.mark.parametrize("name,w,spp", [("cornell555", 48, 32), ("random_spheres", 48, 16), ("ref_cornell_box", 40, 16),
-                                        ("random_spheres_textured", 40, 8), ("box_list", 40, 8),
-                                        ("ref_random_scene", 48, 16), ("cornell_smoke", 40, 16),
-                                        ("ref_two_perlin_spheres", 40, 8), ("ref_two_checker_spheres", 40, 8)])
+KIND_NAMES = {-1: "miss", 0: "lambertian", 1: "metal", 2: "dielectric", 3: "light", 4: "medium"}
+
+
+def primary_kinds(tpt, oracle, name, w, h):
+    """Material kind (TPT_MAT_*, -1 for a miss) of each pixel's centre ray
+    through the lens centre, row 0 at the bottom as in the sums."""
+    hs = tpt.HostScene(name)
+    c = hs.camera.as_array()
+    origin, llc, hor, ver = c[0:3], c[3:6], c[6:9], c[9:12]
+    j, i = np.mgrid[0:h, 0:w]
+    s_ = ((i + 0.5) / w).reshape(-1, 1).astype(np.float32)
+    t_ = ((j + 0.5) / h).reshape(-1, 1).astype(np.float32)
+    d = llc + s_ * hor + t_ * ver - origin
+    rays = np.concatenate([np.broadcast_to(origin, d.shape), d], 1).T.astype(np.float32)
+    _, _, _, mat = oracle.scene(name).trace(np.ascontiguousarray(rays), record=True)
+    kinds = hs.material_kinds().astype(np.int64)
+    return np.where(mat >= 0, kinds[np.maximum(mat, 0)], -1).reshape(h, w)
+
+
+def agreement_by_material(close, kinds):
+    """Share of pixels equal to rounding, over all pixels and per material
+    kind of the primary hit (kinds with at least 100 pixels)."""
+    out = {"all": float(close.mean())}
+    for k in np.unique(kinds):
+        m = kinds == k
+        if m.sum() >= 100:
+            out[KIND_NAMES[int(k)]] = float(close[m].mean())
+    return out
+
+
+# Same-stream agreement: pixels equal to rounding over the image and per
+# material class of the primary hit.  Measured on MI355X: 1.0 for every scene
+# but ref_cornell_box (0.9994: one pixel of 1600, a lambertian one, 0.998 of
+# that class); the bounds allow about twice that.
+SAME_STREAM_SCENES = [("cornell555", 48, 32), ("random_spheres", 48, 16), ("ref_cornell_box", 40, 16),
+                      ("random_spheres_textured", 40, 8), ("box_list", 40, 8), ("ref_random_scene", 48, 16),
+                      ("cornell_smoke", 40, 16), ("ref_two_perlin_spheres", 40, 8), ("ref_two_checker_spheres", 40, 8)]
+
+
+@pytest.mark.parametrize("name,w,spp", SAME_STREAM_SCENES)
 def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
     s_g, q_g, st = gpu_render(tpt, name, w, w, spp, seed=11)
     s_c, q_c = oracle.scene(name).render(w, w, spp, seed=11)
     assert st["paths"] == w * w * spp
-    close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
-    frac = close.all(axis=2).mean()
-    print(f"{name}: {frac:.4f} of pixels equal to rounding; mean gpu {s_g.mean():.5f} cpu {s_c.mean():.5f}")
-    assert frac > 0.9
+    close = (np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)).all(axis=2)
+    agree = agreement_by_material(close, primary_kinds(tpt, oracle, name, w, w))
+    print(f"{name}: pixels equal to rounding {agree}; mean gpu {s_g.mean():.5f} cpu {s_c.mean():.5f}")
+    assert agree["all"] >= 0.998
+    for k, v in agree.items():
+        assert v >= 0.99, (k, v)
     z = zscores(s_g, q_g, spp, s_c, q_c, spp)
-    assert np.mean(np.abs(z) > 4) < 0.01
+    assert np.mean(np.abs(z) > 4) < 0.005
 
 
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 40, 64), ("random_spheres", 40, 32),
@@ -114,9 +152,11 @@ def test_media_scatter_mode_matches_oracle(tpt, oracle, device):
     s_g, q_g, _ = gpu_render(tpt, "cornell_smoke", w, w, spp, seed=21, media_scatter=True)
     s_c, q_c = oracle.scene("cornell_smoke").render(w, w, spp, seed=21, media_scatter=True)
     close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
-    assert close.all(axis=2).mean() > 0.9
+    frac = close.all(axis=2).mean()
+    print(f"media scatter: {frac:.4f} of pixels equal to rounding")
+    assert frac >= 0.998
     z = zscores(s_g, q_g, spp, s_c, q_c, spp)
-    assert np.mean(np.abs(z) > 4) < 0.01
+    assert np.mean(np.abs(z) > 4) < 0.005
     s_b, _, _ = gpu_render(tpt, "cornell_smoke", w, w, spp, seed=21)
     assert s_g.mean() > 1.02 * s_b.mean()
 
@@ -134,6 +174,6 @@ def test_earth_scenes_match_oracle(tpt, oracle, device, earth_cwd, name, w, spp)
     close = np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)
     frac = close.all(axis=2).mean()
     print(f"{name}: {frac:.4f} of pixels equal to rounding")
-    assert frac > 0.85
+    assert frac >= 0.995  # measured 1.0 and 0.9990 (one pixel of 1024)
     z = zscores(s_g, q_g, spp, s_c, q_c, spp)
-    assert np.mean(np.abs(z) > 4) < 0.02
+    assert np.mean(np.abs(z) > 4) < 0.005

@@ -164,3 +164,62 @@ def test_media_free_path_law(oracle):
     mean_trunc = 100.0 - chord * np.exp(-0.01 * chord) / (1 - np.exp(-0.01 * chord))
     se = np.sqrt(np.sum(100.0 ** 2 * np.ones_like(x))) / len(x)
     assert abs(x.mean() - mean_trunc.mean()) < 5 * se, (x.mean(), mean_trunc.mean())
+
+
+SKY_TOP = np.float32([0.5, 0.7, 1.0])  # the book's gradient (commented out at src/utils.cc:87-90)
+
+
+def test_sky_gradient_closed_forms(oracle):
+    """A miss returns the sky gradient times `background` (this build's
+    extension for scenes without lights): straight up the blue end, straight
+    down white, horizontal the midpoint, independent of the direction's
+    length and of x/z, linear in the background scale."""
+    s = oracle.scene("random_spheres")
+    up, down = [0.0, 5000.0, 0.0], [0.0, -3000.0, 0.0]  # above / below everything
+    rays = np.float32([up + [0, 1, 0], up + [0, 2.5, 0], up + [1, 0, 0], up + [0, 0, -3], down + [0, -1, 0],
+                       down + [0, -4, 0], up + [3, 4, 0], up + [0, 4, 3]]).T
+    c = s.color_rays(rays, background=1.0)
+    assert np.array_equal(c[:, 0], SKY_TOP) and np.array_equal(c[:, 1], SKY_TOP)
+    mid = np.float32(0.5) * (np.float32(1) + SKY_TOP)
+    assert np.allclose(c[:, 2], mid, atol=1e-7) and np.allclose(c[:, 3], mid, atol=1e-7)
+    assert np.array_equal(c[:, 4], np.ones(3, np.float32)) and np.array_equal(c[:, 5], np.ones(3, np.float32))
+    t = np.float32(0.9)  # d.y = 0.8 for (3, 4, 0) and (0, 4, 3)
+    assert np.allclose(c[:, 6], (1 - t) + t * SKY_TOP, atol=1e-6) and np.allclose(c[:, 6], c[:, 7], atol=1e-7)
+    assert np.array_equal(s.color_rays(rays, background=0.25), c * np.float32(0.25))
+
+
+def test_cosine_only_lambertian_closed_form(oracle):
+    """Scenes without a light list sample lambertian surfaces with cosine_pdf
+    alone, weighting atten * scattering_pdf / value = atten * (cos/pi) / cos:
+    the reference's cosine_pdf value has no 1/pi (headers/utils.h:72-88).
+    On the ground sphere of random_spheres (albedo 0.5), at points whose
+    tangent plane hides every other sphere, each bounce sees only the sky,
+    so E[L] = (0.5/pi) * E_cos[sky(d)], and for cosine-distributed d about
+    the normal n, E[d] = (2/3) n: E[L] = (0.5/pi) * ((1 - tb) + tb * top)
+    with tb = (1 + (2/3) n_y) / 2.  Checked within 5 standard errors on
+    2^17 samples per point, plus the share of grazing bounces that re-hit
+    the sphere they leave (the float hit point at coordinates near 1000 can
+    lie inside it; t_min is 0.001, src/utils.cc:61): the blue end of the
+    gradient is 1, so a bounce that sees only sky returns exactly 0.5/pi in
+    blue, and the others are counted.  A missing or doubled 1/pi (x3.1), a
+    uniform instead of cosine-weighted hemisphere (2% at these normals) or a
+    flipped normal all fail it."""
+    s = oracle.scene("random_spheres")
+    n_rays = 1 << 17
+    centre = np.array([0.0, -1000.0, 0.0])
+    for phi, ny_raw in [(0.3, -1.0), (1.9, -0.5), (3.5, 0.0), (4.4, 0.5), (5.8, 1.5)]:
+        n = np.array([np.cos(phi), ny_raw, np.sin(phi)])
+        n /= np.linalg.norm(n)
+        p = centre + 1000.0 * n
+        grid_dir = -p / np.linalg.norm(p)  # the other spheres sit within a few units of the origin
+        assert grid_dir @ n < -0.2
+        o = (p + 5.0 * n).astype(np.float32)
+        rays = np.repeat(np.concatenate([o, (-n).astype(np.float32)])[:, None], n_rays, 1)
+        c = s.color_rays(rays, max_depth=50, background=1.0, seed=77)
+        tb = 0.5 * (1 + (2.0 / 3.0) * n[1])
+        expect = (0.5 / np.pi) * ((1 - tb) + tb * SKY_TOP.astype(np.float64))
+        rehit = np.mean(np.abs(c[2] - np.float32(0.5 / np.pi)) > 1e-6)
+        assert rehit < 2e-3, rehit
+        mean = c.mean(axis=1)
+        se = c.std(axis=1) / np.sqrt(n_rays)
+        assert np.all(np.abs(mean - expect) < 5 * se + rehit * expect + 1e-7), (n, mean, expect, se, rehit)

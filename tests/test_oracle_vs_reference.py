@@ -1,7 +1,26 @@
 """The oracle against the reference's own code compiled from its sources
-(oracle/_ref), on fresh seeded rays: identical hits, bit for bit."""
+(oracle/_ref): identical hits on fresh seeded rays, identical samplers,
+light densities, textures and metal scattering, and identical color()
+samples on whole scenes, bit for bit.
+
+Where the reference draws (drand_r: a thread_local std::mt19937 from its
+default seed), it runs on a fresh thread and the oracle replays the same
+generator.  The reference takes the coordinates of some random vectors
+inside one constructor call, vec3(drand_r(), drand_r(), ...)
+(src/utils.cc:16, 24; src/rect_box.cc:40-41; src/hitable.cc:120), whose
+argument order C++ leaves unspecified; the reference build evaluates them
+last argument first (test_reference_draws_constructor_arguments_last_first),
+and the oracle's generator replays that order (rtl=1)."""
+import importlib
+
 import numpy as np
 import pytest
+
+RTL = 1  # the reference build's argument order, established below
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 PAIRS = [("cornell555", "cornell555"), ("random_spheres", "random_spheres"),
          ("random_spheres_textured", "random_spheres_textured"),
@@ -33,12 +52,23 @@ def test_reference_render_runs(reference):
     assert np.isfinite(img).all() and img.sum() > 0
 
 
-def test_oracle_color_matches_reference_color_statistically(oracle, reference):
-    """color() restated vs the reference's color(): per-pixel means within 3 sigma."""
+def cam_params(tpt, name, aspect=1.0):
+    i = tpt.HostScene(name).info
+    return [*i.lookfrom, *i.lookat, 0, 1, 0, i.vfov, aspect, i.aperture, i.focus, i.time0, i.time1]
+
+
+@pytest.mark.parametrize("mine,ref", [("cornell555", "cornell555"), ("ref_cornell_box", "native:cornell_box"),
+                                      ("ref_sphere_cornell_box", "native:sphere_cornell_box")])
+def test_oracle_color_matches_reference_color_statistically(tpt, oracle, reference, mine, ref):
+    """color() restated vs the reference's color() on independent streams
+    (the oracle's Philox streams, the reference's decorrelated row threads):
+    per-pixel means within 3 sigma.  ref_cornell_box has the metal box and
+    the reference driver's light list (a rect and the r=120 sphere, whose
+    direction is NaN from inside it); ref_sphere_cornell_box a metal sphere."""
     w, spp = 32, 64
-    s_o, q_o = oracle.scene("cornell555").render(w, w, spp, seed=4242)
-    h = reference.scene("cornell555")
-    camp = [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0]
+    s_o, q_o = oracle.scene(mine).render(w, w, spp, seed=4242)
+    h = reference.scene(ref)
+    camp = cam_params(tpt, mine)
     s_r, q_r = reference.render(h, camp, w, w, spp, 50, decorrelate_rows=True, with_sumsq=True)
     m_o, m_r = s_o / spp, s_r / spp
     v = (np.maximum(q_o / spp - m_o ** 2, 0) + np.maximum(q_r / spp - m_r ** 2, 0)) / spp
@@ -132,3 +162,124 @@ def test_oneweek_final_matches_reference(oracle, reference, tmp_path, monkeypatc
         for m in media:
             na, nb = (a[0] == m).sum(), (b[0] == m).sum()
             assert abs(na - nb) <= 4 * np.sqrt(na + nb + 1), (kind, m, na, nb)
+
+
+def test_reference_draws_constructor_arguments_last_first(oracle, reference):
+    """random_in_unit_sphere and random_in_unit_disk (src/utils.cc:13-27):
+    the reference's samples equal the oracle's with the draws taken last
+    coordinate first, and differ with first coordinate first."""
+    for kind in (0, 1):
+        ref = reference.sampler(kind, 1 << 14)
+        assert np.array_equal(bits(oracle.sampler(kind, 1 << 14, RTL)), bits(ref)), kind
+        assert not np.array_equal(bits(oracle.sampler(kind, 1 << 14, 1 - RTL)), bits(ref)), kind
+        r2 = (ref[:, :2] ** 2).sum(1) if kind == 1 else (ref ** 2).sum(1)
+        assert (r2 < 1).all() and (kind == 0 or (ref[:, 2] == 0).all())
+
+
+LIGHTS = {  # kind: (params, a point on or in the shape)
+    0: ([120.0, -50.0, 40.0, 120.0], [120.0, -50.0, 40.0]),   # sphere (the reference driver's r=120 light)
+    1: ([213.0, 343.0, 227.0, 332.0, 554.0], [278.0, 554.0, 280.0]),  # xz_rect (the 555 box's light)
+    2: ([0.0], [60.0, 120.0, -40.0]),                          # the reference driver's list: rect + sphere
+}
+
+
+@pytest.mark.parametrize("kind", sorted(LIGHTS))
+def test_light_pdf_and_random_equal_reference(oracle, reference, kind):
+    """hitable::pdf_value and hitable::random of a sphere light
+    (src/sphere.cc:93-120), an xz_rect light (src/rect_box.cc:26-43) and a
+    light list (src/hitable_list.cc:24-36) on 2^16 origins: outside the
+    shape, inside the sphere (NaN cone, pdf 0), on the rect's plane;
+    directions aimed at the shape, at random and axis-aligned."""
+    params, centre = LIGHTS[kind]
+    rng = np.random.default_rng(40 + kind)
+    n = 1 << 16
+    o = rng.uniform(-400, 700, (n, 3))
+    o[: n // 8] = np.asarray(centre) + rng.normal(0, 40, (n // 8, 3))  # near / inside the shape
+    o[n // 8: n // 8 + 64, 1] = 554.0 if kind == 1 else 298.0         # on the rect's plane
+    d = rng.normal(0, 1, (n, 3))
+    aim = n // 2
+    d[:aim] = np.asarray(centre) + rng.normal(0, 70, (aim, 3)) - o[:aim]
+    d[-64:] = np.repeat(np.eye(3), 22, 0)[:64]
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    p_ref, p_or = reference.light_pdf(kind, params, o, d), oracle.light_pdf(kind, params, o, d)
+    assert np.array_equal(bits(p_or), bits(p_ref))
+    assert 0.1 < (p_ref > 0).mean() < 0.9
+    r_ref, r_or = reference.light_random(kind, params, o), oracle.light_random(kind, params, o, RTL)
+    assert np.array_equal(bits(r_or), bits(r_ref))
+    if kind == 0:
+        inside = np.linalg.norm(o - np.float32(centre), axis=1) < params[3] * 0.999
+        assert inside.sum() > 100 and np.isnan(r_ref[inside]).any(axis=1).all()
+
+
+def test_textures_equal_reference(oracle, reference):
+    """checker_texture::value (src/texture.cc:4-16) on 2^16 points, including
+    points beside the zeros of sin(10 x) where the sign flips, and
+    image_texture::value (src/texture.cc:27-42) on (u, v) across and beyond
+    [0, 1], at texel edges and exactly on 0 and 1."""
+    rng = np.random.default_rng(11)
+    n = 1 << 16
+    q = np.zeros((n, 5), np.float32)
+    q[:, :2] = rng.uniform(-0.3, 1.3, (n, 2))
+    q[:, 2:] = rng.uniform(-4, 4, (n, 3))
+    k = rng.integers(-12, 13, (n // 4, 3))
+    q[: n // 4, 2:] = (k * np.pi / 10 + rng.choice([-1e-6, 0, 1e-6], (n // 4, 3))).astype(np.float32)
+    cols = [0.2, 0.3, 0.1, 0.9, 0.9, 0.9]
+    a, b = oracle.texture_value(0, cols, None, q), reference.texture_value(0, cols, None, q)
+    assert np.array_equal(bits(a), bits(b))
+    odd = (a == np.float32([0.2, 0.3, 0.1])).all(1)
+    assert 0.3 < odd.mean() < 0.7
+    w, h = 53, 37
+    tex = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    edges = np.array([0.0, 1.0, -1e-7, 1 + 1e-7, 0.5, 1 / w, 1 / h, 1 - 1 / w, 2.0, -2.0], np.float32)
+    q[:100, 0] = np.resize(edges, 100)
+    q[:100, 1] = np.resize(edges[::-1], 100)
+    q[100:100 + w, 0] = np.arange(w) / np.float32(w)
+    a, b = oracle.texture_value(1, None, tex, q), reference.texture_value(1, None, tex, q)
+    assert np.array_equal(bits(a), bits(b))
+    assert len(np.unique(a[:, 0])) > 200
+
+
+@pytest.mark.parametrize("fuzz", [0.0, 0.3, 0.99, 1.5])
+def test_metal_scatter_equals_reference(oracle, reference, fuzz):
+    """metal::scatter (src/material.cc:88-98; a fuzz of 1 or more becomes 1,
+    headers/material.h:41-46) on 2^16 incoming rays and records: the
+    scattered direction and the absorbed flag."""
+    rng = np.random.default_rng(int(fuzz * 100))
+    q = rng.normal(0, 1, (1 << 16, 13)).astype(np.float32)
+    q[:, 10:13] /= np.linalg.norm(q[:, 10:13], axis=1, keepdims=True)
+    d_ref, ok_ref = reference.metal_scatter([0.7, 0.6, 0.5], fuzz, q)
+    d_or, ok_or = oracle.metal_scatter([0.7, 0.6, 0.5], fuzz, q, RTL)
+    assert np.array_equal(bits(d_or), bits(d_ref)) and np.array_equal(ok_or, ok_ref)
+    assert 0.3 < ok_ref.mean() < 0.7
+
+
+PIXEL_SCENES = [("cornell555", "cornell555"), ("ref_cornell_box", "native:cornell_box"),
+                ("ref_sphere_cornell_box", "native:sphere_cornell_box"), ("ref_light_spheres", "ref_light_spheres"),
+                ("cornell_smoke", "cornell_smoke")]
+
+
+@pytest.mark.parametrize("mine,ref", PIXEL_SCENES)
+def test_color_equals_reference_per_sample(tpt, oracle, reference, mine, ref):
+    """The driver's sample loop (main.cpp:115-126: u, v, get_ray, color(),
+    de_nan) on 256 pixels x 32 samples, each pixel from a fresh generator on
+    both sides: every sample's colour is the reference's bit for bit.  This
+    pins color() (src/utils.cc:58-94) with everything it reaches on these
+    scenes: lambertian with mixture_pdf(hitable_pdf, cosine_pdf), rect and
+    sphere lights (the list of the reference driver), metal, dielectric,
+    diffuse_light, Perlin textures (the oracle's tables installed in the
+    reference's statics), constant_medium's sequential draws, the thin-lens
+    camera."""
+    W = H = 96
+    s = oracle.scene(mine)
+    h = reference.scene(ref)
+    if mine == "ref_light_spheres":
+        reference.perlin_install(*oracle.perlin_tables())
+    cam = cam_params(tpt, mine)
+    rng = np.random.default_rng(sum(map(ord, mine)))
+    pix = np.stack([rng.integers(0, W, 256), rng.integers(0, H, 256)], 1)
+    a = s.pixel_samples(cam, W, H, pix, 32, 50, RTL)
+    b = reference.pixel_samples(h, cam, W, H, pix, 32, 50)
+    assert np.array_equal(bits(a), bits(b))
+    lit = (a.sum(axis=2) > 0).mean()
+    assert lit > 0.002, f"only {lit:.4f} of the samples carry light"
+

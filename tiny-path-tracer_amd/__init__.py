@@ -181,6 +181,14 @@ class HostScene:
         self.info = SceneInfo()
         _check(L.tpt_host_scene_info(h, C.byref(self.info)), "tpt_host_scene_info")
 
+    def material_kinds(self):
+        """Kind (TPT_MAT_*) of every flattened material, by material index."""
+        n = self.flat.n_materials
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        words = np.ctypeslib.as_array(C.cast(self.flat.materials, C.POINTER(C.c_uint32)), shape=(n * 8,))
+        return words[0::8].copy()
+
     def slots(self):
         n = self.flat.n_slots
         arr = (_Slot * n).from_address(self.flat.slots) if n else []
