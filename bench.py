@@ -31,6 +31,13 @@ HBM_PEAK_GBS = 8000.0
 TRACE_BYTES_PER_SEGMENT = 48
 SHADE_BYTES_PER_SEGMENT = 116
 ACCUM_BYTES_PER_PATH = 24
+# Counted VALU work of the closest-hit walk, lane-operations per operation:
+# a slab test 18, a sphere test 30 (its FP64 roots apart), a rect test 14, a
+# wrapper chain (translate + rotate_y) 25.  Peak: the non-packed FP32 lane
+# rate, 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
+LANE_OPS = {"box_tests": 18, "sphere_tests": 30, "rect_tests": 14, "xforms": 25}
+VALU_PEAK_TLANE_OPS = 39.32
+COUNT_SPP = 64
 
 
 def parse():
@@ -39,7 +46,10 @@ def parse():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--spp", type=int, default=None, help="samples per pixel per step (default: the scene's 2048)")
-    p.add_argument("--cpu-spp", type=int, default=256, help="spp of the bounded CPU baseline sample (about 15 s)")
+    p.add_argument("--cpu-spp", type=int, default=512,
+                   help="spp of the bounded CPU baseline sample: the reference's render with snapshots every "
+                        "--cpu-every spp (about 30 s at 1200x1200); its snapshots give the CPU time to RMSE 0.01")
+    p.add_argument("--cpu-every", type=int, default=32)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--scene", default="cornell555")
     p.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
@@ -85,14 +95,20 @@ def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=SNAPSHOT_SPP, max_sp
         pts.append((r.spp, t, e))
         if (e <= RMSE_TARGET and k >= train_passes) or r.spp >= max_spp:
             break
+    return pts, crossing(pts)
+
+
+def crossing(pts):
+    """Time where RMSE crosses the target between bracketing points (spp, t,
+    rmse), RMSE^2 linear in 1/t; None if it does not."""
     for (s0, t0, e0), (s1, t1, e1) in zip(pts, pts[1:]):
         if e0 > RMSE_TARGET >= e1:
             x0, x1, y0, y1 = 1 / t0, 1 / t1, e0 * e0, e1 * e1
-            return pts, 1 / (x0 + (RMSE_TARGET ** 2 - y0) * (x1 - x0) / (y1 - y0))
-    return pts, None
+            return 1 / (x0 + (RMSE_TARGET ** 2 - y0) * (x1 - x0) / (y1 - y0))
+    return None
 
 
-def time_to_rmse(tpt, torch, sc, ref_spp, cpu_rate):
+def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
     W, H = sc.host.info.width, sc.host.info.height
     ref_sum, ref_sq, _ = sc.render(spp=ref_spp, seed=987654321)
     ref = torch.clamp(torch.sqrt(ref_sum / ref_spp), 0.0, 1.0)
@@ -118,13 +134,43 @@ def time_to_rmse(tpt, torch, sc, ref_spp, cpu_rate):
            "guided_speedup": round(t_plain / t_guided, 3) if t_plain and t_guided else None,
            "guided_schedule": sch, "guided_mix_weight": round(alpha, 3),
            "snapshots_every_spp": SNAPSHOT_SPP}
-    if cpu_rate:
-        # same estimator, so the same samples per pixel; at the CPU's measured rate
-        out["cpu_s_estimated"] = round(spp_plain * W * H / (cpu_rate * 1e6), 1)
+    if cpu is not None:
+        # the reference's own render with its progressive snapshots: each
+        # snapshot's RMSE, at the time the render took for its samples
+        snaps, every, secs = cpu["snapshots"], cpu["every"], cpu["seconds"]
+        n = snaps.shape[0]
+        pts = []
+        for k in range(n):
+            s_k = torch.from_numpy(snaps[k]).cuda()
+            pts.append(((k + 1) * every, secs * (k + 1) / n, tpt.display_rmse(s_k, (k + 1) * every, ref)))
+        t_cpu = crossing(pts)
+        out["cpu_s"] = round(t_cpu, 2) if t_cpu else None
+        out["cpu_curve"] = [[p[0], round(p[1], 2), round(p[2], 5)] for p in pts]
+        out["cpu_how"] = (f"the reference's own render (oracle/_ref) of {W}x{H} at {n * every} spp with its "
+                          f"progressive snapshots every {every} spp (main.cpp:111-133), {secs:.1f} s on "
+                          f"{cpu['cores']} hardware threads; a snapshot's time is the render's time for its samples")
+        if t_cpu and t_plain:
+            out["plain_gpu_x_cpu"] = round(t_cpu / t_plain, 1)
     return out
 
 
-def cpu_baseline(scene, w, h, spp, depth):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scene, w, h, spp, every, depth):
+    """The bounded CPU sample: the reference's own color() and thread-per-row
+    loop (oracle/_ref) on all host threads, with its progressive snapshots
+    (kept for the CPU time to RMSE), and the same loop one row thread at a
+    time for the single-thread rate.  Without the reference build: the CPU
+    oracle's restatement (no snapshots)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import REF_SO, Oracle, Reference
 
@@ -132,12 +178,18 @@ def cpu_baseline(scene, w, h, spp, depth):
     cam = {
         "cornell555": [278, 278, -800, 278, 278, 0, 0, 1, 0, 40, 1, 0, 10, 0, 0],
     }.get(scene)
+    snaps = None
+    rate1 = None
     if os.path.exists(REF_SO) and cam is not None:
         R = Reference()
         hnd = R.scene(scene)
         t0 = time.perf_counter()
-        R.render(hnd, cam, w, h, spp, depth, decorrelate_rows=False)
+        snaps = R.render_snapshots(hnd, cam, w, h, spp, every, depth)
         dt = time.perf_counter() - t0
+        spp1 = 2
+        t0 = time.perf_counter()
+        R.render_snapshots(hnd, cam, w, h, spp1, spp1, depth, max_threads=1)
+        rate1 = w * h * spp1 / (time.perf_counter() - t0) / 1e6
         kind = "reference"
         how = "the reference's own color() and thread-per-row loop (oracle/_ref, g++ -O3)"
     else:
@@ -148,9 +200,16 @@ def cpu_baseline(scene, w, h, spp, depth):
         dt = time.perf_counter() - t0
         kind = "port"
         how = "the CPU oracle's color() restatement, per-row mt19937"
-    return {"value": round(w * h * spp / dt / 1e6, 4), "unit": "Msamples/s", "cores": cores, "kind": kind,
-            "sample": f"{scene} {w}x{h} x {spp} spp, depth {depth}: {how}; {dt:.1f} s on {cores} hardware threads "
-                      f"(one std::async task per image row)"}
+    out = {"value": round(w * h * spp / dt / 1e6, 4), "unit": "Msamples/s", "cores": cores, "kind": kind,
+           "cpu_model": cpu_model(),
+           "single_thread_value": round(rate1, 4) if rate1 else None,
+           "sample": f"{scene} {w}x{h} x {spp} spp, depth {depth}: {how}; {dt:.1f} s on {cores} hardware threads "
+                     f"(one std::async task per image row)" +
+                     (f"; single-thread rate from {w}x{h} x 2 spp, one row thread at a time" if rate1 else "")}
+    extra = None
+    if snaps is not None:
+        extra = {"snapshots": snaps, "every": every, "seconds": dt, "cores": cores}
+    return out, extra
 
 
 BATCH_SEED = 2026
@@ -470,7 +529,7 @@ def main():
                        round(totals["nodes_visited"] / max(segs, 1), 2), "trace_ms": round(totals["trace_ms"], 1),
                    "shade_ms": round(totals["shade_ms"], 1), "nan_samples": totals["nan_samples"],
                    "tail_segments": totals["tail_segments"], "tail_ms": round(totals["tail_ms"], 1),
-                   "tail_launches": totals["tail_launches"],
+                   "segments": segs, "tail_launches": totals["tail_launches"],
                    # BVH / primitive bytes the walk requests: one 32-byte slot per visit (boxes and
                    # leaves alike), served from L1/L2 (the Cornell box's slots are 576 bytes)
                    "bvh_slot_bytes_per_segment": round(32 * totals["nodes_visited"] / max(segs, 1), 1),
@@ -479,13 +538,39 @@ def main():
                                                ((totals["trace_ms"] + totals["tail_ms"]) / 1e3) / 1e9, 1)
                    if totals["trace_ms"] else None},
     }
+    # counted VALU work of the walk: one untimed render of the same frame at
+    # COUNT_SPP samples per pixel with the counting trace kernel (every segment
+    # through k_trace), per segment, times the timed run's bounce-loop segments
+    _, _, cst = sc.render(spp=COUNT_SPP, seed=4242, sum_buf=sum_buf, sumsq_buf=sq_buf, count_ops=True)
+    csegs = max(cst["segments"], 1)
+    per_seg = {k: cst[k] / csegs for k in LANE_OPS}
+    lane_ops_per_seg = sum(LANE_OPS[k] * v for k, v in per_seg.items())
+    valu = {"bound": "valu", "kernel": "k_trace", "unit": "Tlane-ops/s", "peak": VALU_PEAK_TLANE_OPS,
+            "counted_lane_ops_per_segment": round(lane_ops_per_seg, 2),
+            "ops_per_segment": {k: round(v, 4) for k, v in per_seg.items()},
+            "weights": LANE_OPS}
+    if totals["trace_ms"] > 0:
+        ach = lane_ops_per_seg * loop_segs / (totals["trace_ms"] / 1e3) / 1e12
+        valu.update(achieved=round(ach, 3), frac=round(ach / VALU_PEAK_TLANE_OPS, 4))
+    # executed VALU lane-operations per segment of the same kernel, from the
+    # gfx950 counters (SQ_INSTS_VALU x active lanes) of a committed profile
+    vfile = os.path.join(ROOT, "profiles", f"{args.scene}_{W}_plain_valu.json")
+    if os.path.exists(vfile) and totals["trace_ms"] > 0:
+        with open(vfile) as f:
+            vk = json.load(f)["kernels"].get("k_trace")
+        if vk:
+            ex = vk["executed_lane_ops_per_segment"] * loop_segs / (totals["trace_ms"] / 1e3) / 1e12
+            valu.update(executed_lane_ops_per_segment=vk["executed_lane_ops_per_segment"],
+                        executed=round(ex, 3), executed_frac=round(ex / VALU_PEAK_TLANE_OPS, 4),
+                        executed_source=f"profiles/{args.scene}_{W}_plain_valu.json")
+    out["roofline"]["valu"] = valu
+    cpu_extra = None
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.scene, W, H, args.cpu_spp, depth)
+        out["cpu_baseline"], cpu_extra = cpu_baseline(args.scene, W, H, args.cpu_spp, args.cpu_every, depth)
         out["detail"]["x_host_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     if not args.no_rmse:
         del sum_buf, sq_buf
-        out["time_to_rmse"] = time_to_rmse(tpt, torch, sc, args.ref_spp,
-                                           out.get("cpu_baseline", {}).get("value"))
+        out["time_to_rmse"] = time_to_rmse(tpt, torch, sc, args.ref_spp, cpu_extra)
     print(json.dumps(out), flush=True)
 
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TPT_ABI_VERSION 1
+#define TPT_ABI_VERSION 2
 
 enum tpt_status {
   TPT_OK = 0,
@@ -347,6 +347,11 @@ typedef struct tpt_render_desc {
  * constant textures, at most one rect light, no motion or media, on the
  * small-scene or sphere-only walk); every sample is the same bit for bit. */
 #define TPT_RENDER_NO_TAIL 4u
+/* flags: count the closest-hit walk's operations by kind (box_tests,
+ * sphere_tests, rect_tests, xforms of tpt_render_stats), for the counted VALU
+ * work of a render.  Runs a counting variant of the trace kernel and no fused
+ * tail; the sums are the same bit for bit. */
+#define TPT_RENDER_COUNT_OPS 8u
 
 typedef struct tpt_render_stats {
   uint64_t paths;          /* camera paths completed */
@@ -360,6 +365,9 @@ typedef struct tpt_render_stats {
   uint64_t tail_segments;  /* segments traced by the fused tail launches (included in segments) */
   double tail_ms;          /* with TPT_RENDER_PROFILE: time of the fused tail launches */
   uint64_t tail_launches;
+  /* with TPT_RENDER_COUNT_OPS (else 0): slab tests, sphere tests, rect tests
+   * and wrapper chains (translate / rotate_y) applied by the closest-hit walks */
+  uint64_t box_tests, sphere_tests, rect_tests, xforms;
 } tpt_render_stats;
 
 int tpt_render(tpt_scene *scene, const tpt_camera *camera, const tpt_render_desc *desc, float *d_sum,

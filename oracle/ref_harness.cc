@@ -360,6 +360,50 @@ int ref_render(void *sp, const float *camp, uint32_t w, uint32_t h, uint32_t spp
   return 0;
 }
 
+// The reference's render loop with its progressive "bonus" snapshots
+// (main.cpp:109-133): every pixel's running sum of de_nan(color()) is kept
+// after each `every` samples; snaps holds spp / every images (3·w·h floats
+// each, row 0 = bottom), the last one the full render.  Each row on a fresh
+// thread, as std::async tasks (max_threads <= 0: all at once, as the driver
+// launches them) or at most max_threads threads at a time (1: one thread).
+// Returns 0, or -1 without a light list or when every does not divide spp.
+int ref_render_snapshots(void *sp, const float *camp, uint32_t w, uint32_t h, uint32_t spp, uint32_t every,
+                         uint32_t max_depth, int max_threads, float *snaps) {
+  auto *s = static_cast<ref_scene_t *>(sp);
+  if (!s->lights || every == 0 || spp % every != 0) return -1;
+  camera cam(vec3(camp[0], camp[1], camp[2]), vec3(camp[3], camp[4], camp[5]), vec3(camp[6], camp[7], camp[8]),
+             camp[9], camp[10], camp[11], camp[12], camp[13], camp[14]);
+  const size_t image = static_cast<size_t>(w) * h * 3;
+  auto row = [&](int j) {
+    for (uint32_t i = 0; i < w; i++) {
+      vec3 col(0, 0, 0);
+      for (uint32_t k = 0; k < spp; k++) {
+        const float u = ((float)i + drand_r()) / (float)w;
+        const float v = ((float)j + drand_r()) / (float)h;
+        ray r = cam.get_ray(u, v);
+        col += de_nan(color(r, s->render_world, s->lights, 0, static_cast<int>(max_depth)));
+        if ((k + 1) % every == 0) {
+          float *out = snaps + ((k + 1) / every - 1) * image + (static_cast<size_t>(j) * w + i) * 3;
+          for (int c = 0; c < 3; ++c) out[c] = col[c];
+        }
+      }
+    }
+  };
+  if (max_threads <= 0) {
+    std::vector<std::future<void>> futures;
+    for (int j = static_cast<int>(h) - 1; j >= 0; j--)
+      futures.emplace_back(std::async(std::launch::deferred | std::launch::async, row, j));
+    for (auto &f : futures) f.wait();
+  } else {
+    for (int j0 = static_cast<int>(h) - 1; j0 >= 0; j0 -= max_threads) {
+      std::vector<std::thread> ts;
+      for (int j = j0; j > j0 - max_threads && j >= 0; --j) ts.emplace_back(row, j);
+      for (auto &t : ts) t.join();
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"
 
 // ---- the reference's Perlin noise (src/perlin_noise.cc, src/utils.cc:155-225) ----

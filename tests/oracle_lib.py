@@ -323,6 +323,8 @@ class Reference:
         L.ref_pixel_samples.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint64, C.c_uint32,
                                         C.c_uint32, C.c_int, _f32p]
         L.ref_perlin_install.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_render_snapshots.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_uint32, C.c_int, _f32p]
 
     def scene(self, name):
         h = C.c_void_p()
@@ -407,4 +409,15 @@ class Reference:
         grad = np.ascontiguousarray(grad, np.float32)
         perm = np.ascontiguousarray(perm, np.int32)
         self.L.ref_perlin_install(grad.ctypes.data, perm.ctypes.data)
+
+    def render_snapshots(self, h, cam_params, width, height, spp, every, max_depth, max_threads=0):
+        """The reference's render with its progressive snapshots: (spp / every,
+        height, width, 3) running sums after every `every` samples."""
+        n = spp // every
+        out = np.zeros(n * width * height * 3, np.float32)
+        rc = self.L.ref_render_snapshots(h, np.asarray(cam_params, np.float32), width, height, spp, every, max_depth,
+                                         max_threads, out)
+        if rc != 0:
+            raise ValueError("reference snapshots need a light list and spp divisible by every")
+        return out.reshape(n, height, width, 3)
 

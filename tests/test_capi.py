@@ -26,7 +26,7 @@ def test_exports_every_declared_symbol(tpt):
 
 def test_abi_version_and_error_path(tpt):
     L = tpt.lib()
-    assert L.tpt_abi_version() == 1
+    assert L.tpt_abi_version() == 2
     h = C.c_void_p()
     rc = L.tpt_host_scene_build(b"no_such_scene", 1, C.byref(h))
     assert rc == -1 and b"unknown scene" in L.tpt_last_error()

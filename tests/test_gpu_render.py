@@ -177,3 +177,29 @@ def test_earth_scenes_match_oracle(tpt, oracle, device, earth_cwd, name, w, spp)
     assert frac >= 0.995  # measured 1.0 and 0.9990 (one pixel of 1024)
     z = zscores(s_g, q_g, spp, s_c, q_c, spp)
     assert np.mean(np.abs(z) > 4) < 0.005
+
+
+@pytest.mark.parametrize("name", ["cornell555", "random_spheres", "ref_cornell_box"])
+def test_count_ops_mode_changes_nothing_but_the_counts(tpt, device, name):
+    """TPT_RENDER_COUNT_OPS runs the counting trace kernel and no fused tail:
+    the sums are the same bit for bit, and the walk's operations are counted
+    by kind (box slots visited, sphere and rect tests, wrapper chains)."""
+    import torch
+
+    sc = tpt.Scene.named(name)
+    cam = sc.host.camera
+    a, qa, sa = sc.render(width=40, height=40, spp=8, seed=9, camera=cam)
+    b, qb, sb = sc.render(width=40, height=40, spp=8, seed=9, camera=cam, count_ops=True)
+    assert torch.equal(a, b) and torch.equal(qa, qb)
+    assert sa["segments"] == sb["segments"] and sa["box_tests"] == 0
+    assert sb["tail_segments"] == 0 and sb["box_tests"] > 0
+    leaves = sb["sphere_tests"] + sb["rect_tests"]
+    print(name, {k: sb[k] / sb["segments"] for k in ("box_tests", "sphere_tests", "rect_tests", "xforms",
+                                                      "nodes_visited")})
+    # every visited slot is a box or a leaf slot; a list slot holds several leaves
+    assert sb["box_tests"] < sb["nodes_visited"] <= sb["box_tests"] + leaves
+    if name == "random_spheres":
+        assert sb["rect_tests"] == 0 and sb["xforms"] == 0
+        assert sb["nodes_visited"] == sb["box_tests"] + sb["sphere_tests"]
+    else:
+        assert sb["rect_tests"] > 0 and sb["xforms"] > 0

@@ -103,7 +103,8 @@ class RenderStats(C.Structure):
                 ("nan_samples", C.c_uint64), ("kernel_ms", C.c_double), ("nodes_visited", C.c_uint64),
                 ("trace_ms", C.c_double), ("shade_ms", C.c_double), ("trace_launches", C.c_uint64),
                 ("shade_launches", C.c_uint64), ("tail_segments", C.c_uint64), ("tail_ms", C.c_double),
-                ("tail_launches", C.c_uint64)]
+                ("tail_launches", C.c_uint64), ("box_tests", C.c_uint64), ("sphere_tests", C.c_uint64),
+                ("rect_tests", C.c_uint64), ("xforms", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -341,10 +342,11 @@ class Scene:
 
     def render(self, width=None, height=None, spp=None, max_depth=None, seed=1, sample_offset=0, background=None,
                guiding=0, queue_capacity=0, sum_buf=None, sumsq_buf=None, stream=None, camera=None,
-               profile=False, guide=None, guide_vertices=0, media_scatter=False, tail=True):
+               profile=False, guide=None, guide_vertices=0, media_scatter=False, tail=True, count_ops=False):
         """Renders into (and returns) device tensors sum, sumsq of shape (H, W, 3);
         row 0 is the bottom image row as in the reference.  tail=False drains
-        every pass with the bounce loop alone (TPT_RENDER_NO_TAIL)."""
+        every pass with the bounce loop alone (TPT_RENDER_NO_TAIL); count_ops
+        counts the walk's operations by kind (TPT_RENDER_COUNT_OPS)."""
         torch = _torch()
         info = self.host.info
         desc = RenderDesc()
@@ -357,7 +359,7 @@ class Scene:
         desc.background = info.background if background is None else background
         desc.guiding = guiding
         desc.queue_capacity = queue_capacity
-        desc.flags = (1 if profile else 0) | (2 if media_scatter else 0) | (0 if tail else 4)
+        desc.flags = (1 if profile else 0) | (2 if media_scatter else 0) | (0 if tail else 4) | (8 if count_ops else 0)
         desc.guide_vertices = guide_vertices
         shape = (desc.height, desc.width, 3)
         if sum_buf is None:

@@ -38,7 +38,12 @@ namespace tptd {
 // Material queues: lambertian hits (the guided ones in guided mode), every other
 // scattering hit, and hits that end the path (emitters, depth limit).
 enum { Q_DIFFUSE = 0, Q_OTHER = 1, Q_TERMINAL = 2 };
-enum { ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_TAIL = 5, ST_COUNT = 8 };
+// Per-block statistics: the first ST_BASIC counters in every launch, the
+// walk's operations by kind (op_counts) in the counting variant of k_trace.
+enum {
+  ST_SEGMENTS = 0, ST_LIGHT = 1, ST_NAN = 2, ST_PATHS = 3, ST_NODES = 4, ST_TAIL = 5, ST_BASIC = 8,
+  ST_BOX = 8, ST_SPHERE = 9, ST_RECT = 10, ST_XFORM = 11, ST_COUNT = 12
+};
 
 // Every queue is split into NSUB regions, each with its own counter 128 bytes
 // from the next: block b appends to region b % NSUB.  Wave atomics on one
@@ -397,8 +402,9 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
 // code the kernel fits 64 registers, 8 waves per SIMD.  MEDIA: the scenes hold
 // constant_media (their code costs registers, so only this variant has it).  The slot/entry arrays come
 // in as separate restrict arguments so that wave-uniform loads from them can
-// use the scalar data path.
-template <bool SWEEP, bool MULTI, bool SPH = false, bool MEDIA = false>
+// use the scalar data path.  COUNT: also counts the walk's operations by kind
+// (TPT_RENDER_COUNT_OPS; the other variants are built without that code).
+template <bool SWEEP, bool MULTI, bool SPH = false, bool MEDIA = false, bool COUNT = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : (MEDIA ? 1 : 5)))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
@@ -417,7 +423,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
   if (blockIdx.x == 0 && threadIdx.x < NSUB) Q.counters[rc_at(parity ^ 1u, threadIdx.x)] = 0;
   float4 *const *in = Q.rq[parity];
   const uint32_t home = blockIdx.x % NSUB;
-  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
     const uint32_t qi = base + threadIdx.x;
@@ -446,8 +452,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
     const ray_ctx rc = !SWEEP && !SPH && (P.motion | P.media) && valid
                            ? make_ray_ctx(P, item_in_entry(P, ent, __float_as_uint(a.w)), depth)
                            : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u};
+    op_counts oc;
     const bool got = trace_world<SWEEP, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax,
-                                                    &h, &st[ST_NODES], valid);
+                                                    &h, &st[ST_NODES], valid, COUNT ? &oc : nullptr);
+    if (COUNT) {
+      st[ST_BOX] += oc.box;
+      st[ST_SPHERE] += oc.sphere;
+      st[ST_RECT] += oc.rect;
+      st[ST_XFORM] += oc.xform;
+    }
     const float4 c = SWEEP || !valid ? c0 : in[2][src];
     if (valid) {
       ++st[ST_SEGMENTS];
@@ -479,7 +492,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
       }
     }
   }
-  block_stats(Q.stats, st, ST_COUNT);
+  block_stats(Q.stats, st, COUNT ? ST_COUNT : ST_BASIC);
 }
 
 __device__ __forceinline__ float luminance(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
@@ -684,7 +697,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
   const uint32_t home = blockIdx.x % NSUB;
   uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
   float4 *const *out = Q.rq[parity ^ 1u];
-  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
     const uint32_t qi = base + threadIdx.x;
@@ -708,7 +721,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
     }
     claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
   }
-  block_stats(Q.stats, st, ST_COUNT);
+  block_stats(Q.stats, st, ST_BASIC);
 }
 
 // Guided mode, diffuse queue: network inference fused with the guided
@@ -754,7 +767,7 @@ __global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((am
   const uint32_t home = blockIdx.x % NSUB;
   uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
   float4 *const *out = Q.rq[parity ^ 1u];
-  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < total;
@@ -884,7 +897,7 @@ __global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((am
     }
     claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
   }
-  block_stats(Q.stats, st, ST_COUNT);
+  block_stats(Q.stats, st, ST_BASIC);
 }
 
 // Fused tail of a pass.  Once every item of the pass is claimed and few paths
@@ -918,7 +931,7 @@ __global__ void __launch_bounds__(256) k_tail(scene_view S, const tpt_slot *__re
   if (blockIdx.x == 0 && threadIdx.x < NSUB) Q.counters[rc_at(parity ^ 1u, threadIdx.x)] = 0;
   float4 *const *in = Q.rq[parity];
   const uint32_t home = blockIdx.x % NSUB;
-  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
     const uint32_t qi = base + threadIdx.x;
@@ -964,7 +977,7 @@ __global__ void __launch_bounds__(256) k_tail(scene_view S, const tpt_slot *__re
       }
     }
   }
-  block_stats(Q.stats, st, ST_COUNT);
+  block_stats(Q.stats, st, ST_BASIC);
 }
 
 // Each job's sum/sumsq += its samples of the pass, in sample order per pixel.

@@ -698,6 +698,7 @@ struct bounce_launch {
   bool sph;     // every leaf of the launch's scenes is a sphere
   bool simple;  // their textures are constant and their light root absent or one xz_rect
   bool tail;    // passes end with a fused k_tail launch (simple scenes, sweep or sphere-only walk)
+  bool count;   // TPT_RENDER_COUNT_OPS: the counting variant of k_trace, no fused tail
 };
 
 // The fused tail on ray queue `parity` (k_tail), `blocks` blocks.
@@ -727,24 +728,37 @@ void launch_tail(const bounce_launch &B, hipStream_t st, uint32_t parity, uint32
   }
 }
 
-void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
+template <bool COUNT>
+void launch_trace_t(const bounce_launch &B, hipStream_t st, uint32_t parity) {
   const dim3 g(B.grid), b(256);
   if (B.sweep)
-    hipLaunchKernelGGL((k_trace<true, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+    hipLaunchKernelGGL((k_trace<true, false, false, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else if (B.multi && B.P.media)
-    hipLaunchKernelGGL((k_trace<false, true, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
-                       parity);
+    hipLaunchKernelGGL((k_trace<false, true, false, true, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else if (B.P.media)
-    hipLaunchKernelGGL((k_trace<false, false, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q,
-                       parity);
+    hipLaunchKernelGGL((k_trace<false, false, false, true, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else if (B.multi && B.sph)
-    hipLaunchKernelGGL((k_trace<false, true, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+    hipLaunchKernelGGL((k_trace<false, true, true, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else if (B.multi)
-    hipLaunchKernelGGL((k_trace<false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+    hipLaunchKernelGGL((k_trace<false, true, false, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else if (B.sph)
-    hipLaunchKernelGGL((k_trace<false, false, true>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+    hipLaunchKernelGGL((k_trace<false, false, true, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
   else
-    hipLaunchKernelGGL((k_trace<false, false>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P, B.Q, parity);
+    hipLaunchKernelGGL((k_trace<false, false, false, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
+                       B.Q, parity);
+}
+
+void launch_trace(const bounce_launch &B, hipStream_t st, uint32_t parity) {
+  if (B.count)
+    launch_trace_t<true>(B, st, parity);
+  else
+    launch_trace_t<false>(B, st, parity);
 }
 
 void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
@@ -779,7 +793,7 @@ void launch_shade(const bounce_launch &B, hipStream_t st, uint32_t parity) {
 int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_launch &L, tpt_guide *guide,
                 tpt_render_stats *stats, void *stream_ptr) {
   if (!s || (n_jobs && !jobs)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: null argument");
-  if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL))
+  if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL | TPT_RENDER_COUNT_OPS))
     return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
   if (L.queue_capacity > (1u << 27)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^27");
@@ -885,7 +899,8 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   }
   B.multi = multi;
   B.guided = guide != nullptr;
-  B.tail = !(L.flags & TPT_RENDER_NO_TAIL) && B.simple && (B.sweep || B.sph);
+  B.count = (L.flags & TPT_RENDER_COUNT_OPS) != 0;
+  B.tail = !(L.flags & TPT_RENDER_NO_TAIL) && !B.count && B.simple && (B.sweep || B.sph);
   render_params &P = B.P;
   P.jobs = s->d_jobs;
   P.scenes = s->d_scenes;
@@ -1130,6 +1145,10 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     stats->tail_segments = tot[ST_TAIL];
     stats->tail_ms = tail_ms;
     stats->tail_launches = n_tail;
+    stats->box_tests = tot[ST_BOX];
+    stats->sphere_tests = tot[ST_SPHERE];
+    stats->rect_tests = tot[ST_RECT];
+    stats->xforms = tot[ST_XFORM];
   }
   return TPT_OK;
 }

@@ -207,10 +207,21 @@ __device__ __forceinline__ void to_world(const tpt_xform &x, v3 *p, v3 *n) {
   }
 }
 
+// Operations of a walk by kind, for the counted VALU work of a render (the
+// counting variant of k_trace).  Every walk function takes an optional pointer;
+// the other kernels pass a constant null, and with the calls inlined the
+// counting code is removed from them.
+struct op_counts {
+  uint32_t box = 0;     // slab tests (box slots visited)
+  uint32_t sphere = 0;  // sphere and moving-sphere tests
+  uint32_t rect = 0;    // rect tests
+  uint32_t xform = 0;   // wrapper chains applied to a ray (translate / rotate_y)
+};
+
 template <bool MEDIA>
 __device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, uint32_t count, uint32_t leaf0, v3 o,
                                           v3 d, const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
-                                          uint32_t *mat);
+                                          uint32_t *mat, op_counts *oc = nullptr);
 
 // constant_medium::hit (src/hitable.cc:98-128): the boundary's first hit over
 // the whole line, its next hit from there + 0.0001, clipped to (t_min, t_max);
@@ -243,7 +254,7 @@ __device__ __forceinline__ bool medium_t(const scene_view &S, const float *f, ui
 template <bool MEDIA>
 __device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, uint32_t count, uint32_t leaf0, v3 o,
                                           v3 d, const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
-                                          uint32_t *mat) {
+                                          uint32_t *mat, op_counts *oc) {
   const float time = rc.time;
   float cur = tmax;
   bool found = false;
@@ -255,7 +266,12 @@ __device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, u
       lo = o;
       ld = d;
       if (en.xform != kNone) to_local(S.xforms[en.xform], &lo, &ld);
+      if (oc && en.xform != kNone) ++oc->xform;
       cached = en.xform;
+    }
+    if (oc) {
+      if (en.b == TPT_SLOT_SPHERE || en.b == TPT_SLOT_MSPHERE) ++oc->sphere;
+      if (en.b == TPT_SLOT_RECT) ++oc->rect;
     }
     float th;
     bool h;
@@ -281,7 +297,11 @@ __device__ __forceinline__ bool list_eval(const scene_view &S, uint32_t first, u
 template <bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uint32_t kind, v3 o, v3 d,
                                        const ray_ctx &rc, float tmin, float tmax, float *t, uint32_t *entry,
-                                       uint32_t *mat) {
+                                       uint32_t *mat, op_counts *oc = nullptr) {
+  if (oc) {
+    if (SPH || kind == TPT_SLOT_SPHERE || kind == TPT_SLOT_MSPHERE) ++oc->sphere;
+    if (!SPH && kind == TPT_SLOT_RECT) ++oc->rect;
+  }
   if (SPH || kind == TPT_SLOT_SPHERE) {
     *entry = kNone;
     *mat = s.a;
@@ -303,7 +323,7 @@ __device__ __forceinline__ bool leaf_t(const scene_view &S, const slot_t &s, uin
     return medium_t(S, s.f, s.b & TPT_PAYLOAD_MASK, o, d, rc, tmin, tmax, t);
   }
   return list_eval<MEDIA>(S, s.a, __float_as_uint(s.f[1]), __float_as_uint(s.f[0]), o, d, rc, tmin, tmax, t, entry,
-                         mat);
+                         mat, oc);
 }
 
 // Exact replay of the bvh_node recursion for one segment: evaluates each node
@@ -374,7 +394,7 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
 template <bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
                                              const ray_ctx &rc, float tmin, float tmax, hit_t *out, bool *saw_nan,
-                                             uint32_t *nodes) {
+                                             uint32_t *nodes, op_counts *oc = nullptr) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t i = start;
@@ -385,11 +405,12 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
     ++visited;
     uint32_t ni = i + 1;
     if (kind == TPT_SLOT_BOX) {
+      if (oc) ++oc->box;
       if (!slab_nn(s, o, inv, tmin, tmax)) ni = s.a;
     } else {
       float t;
       uint32_t entry, mat;
-      if (leaf_t<SPH, MEDIA>(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat)) {
+      if (leaf_t<SPH, MEDIA>(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat, oc)) {
         if (t != t) *saw_nan = true;
         if (!have || t <= best.t) {
           best = hit_t{t, i, entry, mat};
@@ -424,7 +445,7 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // Must be called by all lanes of the wave together.
 __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t start, uint32_t end, bool live, v3 o, v3 d,
                                               v3 inv, const ray_ctx &rc, float tmin, float tmax, hit_t *out,
-                                              bool *saw_nan, uint32_t *nodes) {
+                                              bool *saw_nan, uint32_t *nodes, op_counts *oc = nullptr) {
   bool have = false;
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t next = live ? start : end;
@@ -437,11 +458,12 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
     if (mine) {
       ++visited;
       if (kind == TPT_SLOT_BOX) {
+        if (oc) ++oc->box;
         next = slab_nn(s, o, inv, tmin, tmax) ? i + 1 : s.a;
       } else {
         float t;
         uint32_t entry, mat;
-        if (leaf_t(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat)) {
+        if (leaf_t(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat, oc)) {
           if (t != t) *saw_nan = true;
           if (!have || t <= best.t) {
             best = hit_t{t, i, entry, mat};
@@ -468,7 +490,7 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
 // updates verbatim.
 template <bool SWEEP = false, bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, const ray_ctx &rc, float tmin, float tmax,
-                                            hit_t *out, uint32_t *nodes, bool live = true) {
+                                            hit_t *out, uint32_t *nodes, bool live = true, op_counts *oc = nullptr) {
   const v3 inv0{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
   const v3 o0 = o, d0 = d;
   float cur = tmax;
@@ -486,15 +508,16 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, con
     const uint32_t sx = !SWEEP && !SPH && S.seg_xf ? S.seg_xf[g] : TPT_SEGMENT_PLAIN;
     if ((sx & TPT_SEGMENT_PLAIN) != TPT_SEGMENT_PLAIN) {
       to_local(S.xforms[sx & TPT_SEGMENT_PLAIN], &o, &d);
+      if (oc) ++oc->xform;
       inv = v3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     }
     const bool bounded = tmin == tmin && cur == cur;
     bool got = false;
     if (SWEEP) {
-      got = segment_sweep(S, start, end, live && bounded, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes);
+      got = segment_sweep(S, start, end, live && bounded, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes, oc);
     } else {
       if (!live) continue;
-      if (bounded) got = segment_fast<SPH, MEDIA>(S, start, end, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes);
+      if (bounded) got = segment_fast<SPH, MEDIA>(S, start, end, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes, oc);
     }
     if (live && (saw_nan || !bounded)) got = segment_exact<SPH, MEDIA>(S, start, o, d, inv, rc, tmin, cur, &h);
     if (got) {
