@@ -76,17 +76,22 @@ GUIDED_SCHEDULE = dict(train_passes=3, pass_spp=4, record_stride=8, iters=64, ba
 SNAPSHOT_SPP = 16
 
 
-def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=SNAPSHOT_SPP, max_spp=8192):
-    """Progressive snapshots: RMSE of the display image after each pass, with
-    the GPU time of the passes (render + training; RMSE evaluation excluded).
-    Returns the points and the interpolated time where RMSE crosses the target
-    (RMSE^2 linear in 1/t between the bracketing snapshots)."""
+def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=4, max_spp=8192):
+    """Progressive snapshots at doubling spp (SNAPSHOT_SPP, 2x, 4x, ...): RMSE
+    of the display image after each pass, with the GPU time of the passes
+    (render + training; RMSE evaluation excluded).  Training passes (guided)
+    come first.  Returns the points (spp, t, rmse) and the crossing of the
+    target (crossing)."""
     t, pts, k = 0.0, [], 0
+    target = SNAPSHOT_SPP
     while True:
         train = k < train_passes
+        if not train:
+            while target <= r.spp:
+                target *= 2
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
-        r.add(pass_spp if train else SNAPSHOT_SPP, train=train)
+        r.add(pass_spp if train else target - r.spp, train=train)
         b.record()
         torch.cuda.synchronize()
         t += a.elapsed_time(b) / 1e3
@@ -99,13 +104,16 @@ def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=SNAPSHOT_SPP, max_sp
 
 
 def crossing(pts):
-    """Time where RMSE crosses the target between bracketing points (spp, t,
-    rmse), RMSE^2 linear in 1/t; None if it does not."""
+    """Where RMSE crosses the target between bracketing snapshots (spp, t,
+    rmse): the samples per pixel n* from RMSE^2 linear in 1/n (the variance of
+    an unbiased estimate), then its time, linear in the samples within the
+    pass that reaches it.  Returns (t*, n*), or (None, None)."""
     for (s0, t0, e0), (s1, t1, e1) in zip(pts, pts[1:]):
         if e0 > RMSE_TARGET >= e1:
-            x0, x1, y0, y1 = 1 / t0, 1 / t1, e0 * e0, e1 * e1
-            return 1 / (x0 + (RMSE_TARGET ** 2 - y0) * (x1 - x0) / (y1 - y0))
-    return None
+            x0, x1, y0, y1 = 1 / s0, 1 / s1, e0 * e0, e1 * e1
+            n = 1 / (x0 + (RMSE_TARGET ** 2 - y0) * (x1 - x0) / (y1 - y0))
+            return t0 + (t1 - t0) * (n - s0) / (s1 - s0), n
+    return None, None
 
 
 def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
@@ -118,22 +126,24 @@ def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
     live = (m > 0) & (torch.sqrt(m) < 1)
     floor = torch.sqrt(torch.where(live, var / (4 * m.clamp(min=1e-12)) / ref_spp, torch.zeros_like(m)).mean()).item()
     del ref_sq, var, m
-    plain_pts, t_plain = rmse_curve(tpt, torch, tpt.ProgressiveRender(sc, seed=1), ref)
+    plain_pts, (t_plain, n_plain) = rmse_curve(tpt, torch, tpt.ProgressiveRender(sc, seed=1), ref)
     g = tpt.Guide(seed=1)
     sch = GUIDED_SCHEDULE
     rg = tpt.ProgressiveRender(sc, guide=g, seed=1, record_stride=sch["record_stride"], iters=sch["iters"],
                                batch=sch["batch"], lr=sch["lr"])
-    guided_pts, t_guided = rmse_curve(tpt, torch, rg, ref, train_passes=sch["train_passes"],
-                                      pass_spp=sch["pass_spp"])
+    guided_pts, (t_guided, n_guided) = rmse_curve(tpt, torch, rg, ref, train_passes=sch["train_passes"],
+                                                  pass_spp=sch["pass_spp"])
     alpha = float(1 / (1 + np.exp(-g.get_params()[-1])))
-    spp_plain = plain_pts[-1][0]
+    r2 = lambda v: round(v, 1) if v else None
     out = {"target": RMSE_TARGET, "on": "clamp(sqrt(mean), 0, 1), all pixels and channels",
            "against": f"plain GPU render, {ref_spp} spp", "reference_rmse_floor": round(floor, 5),
-           "plain_s": round(t_plain, 4) if t_plain else None, "plain_spp": spp_plain,
-           "guided_s": round(t_guided, 4) if t_guided else None, "guided_spp": guided_pts[-1][0],
+           "plain_s": round(t_plain, 4) if t_plain else None, "plain_spp": r2(n_plain),
+           "guided_s": round(t_guided, 4) if t_guided else None, "guided_spp": r2(n_guided),
            "guided_speedup": round(t_plain / t_guided, 3) if t_plain and t_guided else None,
            "guided_schedule": sch, "guided_mix_weight": round(alpha, 3),
-           "snapshots_every_spp": SNAPSHOT_SPP}
+           "snapshots": f"doubling spp from {SNAPSHOT_SPP} (guided: after the training passes)",
+           "plain_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in plain_pts],
+           "guided_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in guided_pts]}
     if cpu is not None:
         # the reference's own render with its progressive snapshots: each
         # snapshot's RMSE, at the time the render took for its samples
@@ -143,8 +153,9 @@ def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
         for k in range(n):
             s_k = torch.from_numpy(snaps[k]).cuda()
             pts.append(((k + 1) * every, secs * (k + 1) / n, tpt.display_rmse(s_k, (k + 1) * every, ref)))
-        t_cpu = crossing(pts)
+        t_cpu, n_cpu = crossing(pts)
         out["cpu_s"] = round(t_cpu, 2) if t_cpu else None
+        out["cpu_spp"] = r2(n_cpu)
         out["cpu_curve"] = [[p[0], round(p[1], 2), round(p[2], 5)] for p in pts]
         out["cpu_how"] = (f"the reference's own render (oracle/_ref) of {W}x{H} at {n * every} spp with its "
                           f"progressive snapshots every {every} spp (main.cpp:111-133), {secs:.1f} s on "

@@ -275,6 +275,16 @@ __device__ __forceinline__ void block_stats(unsigned long long *stats, const uin
 // a miss), and the reference's `emitted + ...` form turns a channel NaN when a
 // non-finite throughput meets a zero term.  So only its NaN bits are stored
 // (bits 16..18 next to the diffuse-vertex count).
+// Queue entries are read once.  Non-temporal loads of them leave L2 to the
+// queues being filled, whose lines arrive in parts (each wave's run starts
+// where the previous one ended) and are written to HBM twice when evicted
+// half-filled.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_once(const float4 *p) {
+  const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
   const uint32_t nan_bits = (p.rad.x != p.rad.x ? 1u : 0u) | (p.rad.y != p.rad.y ? 2u : 0u) |
                             (p.rad.z != p.rad.z ? 4u : 0u);
@@ -283,8 +293,12 @@ __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const p
   q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16)));
 }
 
+// ONCE: the entry's last read (non-temporal); k_guide_shade reads its
+// entries a first time before the network passes.
+template <bool ONCE = true>
 __device__ __forceinline__ path load_ray(float4 *const *q, uint32_t at) {
-  const float4 a = q[0][at], b = q[1][at], c = q[2][at];
+  const float4 a = ONCE ? ld_once(&q[0][at]) : q[0][at], b = ONCE ? ld_once(&q[1][at]) : q[1][at],
+               c = ONCE ? ld_once(&q[2][at]) : q[2][at];
   path p;
   p.o = v3{a.x, a.y, a.z};
   p.item = __float_as_uint(a.w);
@@ -439,9 +453,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
     if (valid) {
       const uint32_t r = find_region(pre, NSUB, qi);
       src = r * Q.region + (qi - pre[r]);
-      a = in[0][src];
-      b = in[1][src];
-      if (SWEEP) c0 = in[2][src];
+      a = ld_once(&in[0][src]);
+      b = ld_once(&in[1][src]);
+      if (SWEEP) c0 = ld_once(&in[2][src]);
     }
     const uint32_t depth = __float_as_uint(b.w) & 0xFFFFu;
     const uint32_t ent = MULTI ? __float_as_uint(b.w) >> 16 : 0u;
@@ -461,7 +475,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
       st[ST_RECT] += oc.rect;
       st[ST_XFORM] += oc.xform;
     }
-    const float4 c = SWEEP || !valid ? c0 : in[2][src];
+    const float4 c = SWEEP || !valid ? c0 : ld_once(&in[2][src]);
     if (valid) {
       ++st[ST_SEGMENTS];
       if (got) {
@@ -709,7 +723,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
       const uint32_t cls = C0 + r / NSUB;
       const uint32_t off = (r % NSUB) * Q.region + (qi - pre[r]);
       p = load_ray(Q.mq[cls], off);
-      const float4 hw = Q.mq[cls][3][off];
+      const float4 hw = ld_once(&Q.mq[cls][3][off]);
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       const item_ctx c = item_in_entry(P, MULTI ? p.ent : 0u, p.item);
       const scene_view Sj = view_of<MULTI>(S, P, c.job);
@@ -781,8 +795,8 @@ __global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((am
     if (valid) {
       const uint32_t r = find_region(pre, NSUB, qi);
       off = r * Q.region + (qi - pre[r]);
-      const path p = load_ray(Q.mq[Q_DIFFUSE], off);
-      const float4 hw = Q.mq[Q_DIFFUSE][3][off];
+      const path p = load_ray<false>(Q.mq[Q_DIFFUSE], off);
+      const float4 hw = ld_once(&Q.mq[Q_DIFFUSE][3][off]);
       const hit_t h{hw.x, __float_as_uint(hw.y), __float_as_uint(hw.z), __float_as_uint(hw.w)};
       hmat = h.mat;
       v3 n;
