@@ -285,12 +285,21 @@ __device__ __forceinline__ float4 ld_once(const float4 *p) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Ray records are stored non-temporally: the next launch reads them long
+// after L2 could hold them, and streaming them past L2 cut k_shade's HBM writes
+// from 61 to 49 B per segment (7% faster).  The material-queue entries of
+// k_trace keep plain stores (non-temporal ones wrote 6 B more per segment and
+// saved no time).
+__device__ __forceinline__ void st_once(float4 *p, float4 v) {
+  __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(p));
+}
+
 __device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
   const uint32_t nan_bits = (p.rad.x != p.rad.x ? 1u : 0u) | (p.rad.y != p.rad.y ? 2u : 0u) |
                             (p.rad.z != p.rad.z ? 4u : 0u);
-  q[0][at] = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
-  q[1][at] = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth | (p.ent << 16)));
-  q[2][at] = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16)));
+  st_once(&q[0][at], make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item)));
+  st_once(&q[1][at], make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth | (p.ent << 16))));
+  st_once(&q[2][at], make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16))));
 }
 
 // ONCE: the entry's last read (non-temporal); k_guide_shade reads its
