@@ -93,13 +93,42 @@ def rmse_curve(tpt, torch, r, ref, train_passes=0, pass_spp=4, max_spp=8192):
         a.record()
         r.add(pass_spp if train else target - r.spp, train=train)
         b.record()
-        torch.cuda.synchronize()
+        b.synchronize()  # this stream only: overlap-mode training may go on beside it
         t += a.elapsed_time(b) / 1e3
         k += 1
         e = tpt.display_rmse(r.sum, r.spp, ref)
         pts.append((r.spp, t, e))
         if (e <= RMSE_TARGET and k >= train_passes) or r.spp >= max_spp:
             break
+    return pts, crossing(pts)
+
+
+def wall_curve(tpt, torch, r, ref, train_passes, pass_spp, stop_spp):
+    """As rmse_curve, for a renderer whose training runs on a stream of its
+    own beside the passes (ProgressiveRender overlap mode): the time of a
+    snapshot is the wall clock from the first pass to the end of its pass,
+    host gaps and concurrent training included, and the RMSE of each snapshot
+    (a copy of the sums taken on the render stream) is evaluated after the
+    last pass, so that no evaluation runs between passes.  Renders to the
+    first doubling target at or above stop_spp."""
+    start = torch.cuda.Event(enable_timing=True)
+    start.record()
+    snaps, k, target = [], 0, SNAPSHOT_SPP
+    while True:
+        train = k < train_passes
+        if not train:
+            while target <= r.spp:
+                target *= 2
+        r.add(pass_spp if train else target - r.spp, train=train)
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        snaps.append((r.spp, e, r.sum.clone()))
+        k += 1
+        if not train and r.spp >= stop_spp:
+            break
+    r.finish()
+    torch.cuda.synchronize()
+    pts = [(s, start.elapsed_time(e) / 1e3, tpt.display_rmse(x, s, ref)) for s, e, x in snaps]
     return pts, crossing(pts)
 
 
@@ -134,6 +163,14 @@ def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
     guided_pts, (t_guided, n_guided) = rmse_curve(tpt, torch, rg, ref, train_passes=sch["train_passes"],
                                                   pass_spp=sch["pass_spp"])
     alpha = float(1 / (1 + np.exp(-g.get_params()[-1])))
+    del rg
+    # the same schedule with training on a second stream beside the passes
+    go = tpt.Guide(seed=1)
+    ro = tpt.ProgressiveRender(sc, guide=go, seed=1, record_stride=sch["record_stride"], iters=sch["iters"],
+                               batch=sch["batch"], lr=sch["lr"], overlap=True)
+    over_pts, (t_over, n_over) = wall_curve(tpt, torch, ro, ref, sch["train_passes"], sch["pass_spp"],
+                                            2 * (n_guided or 256))
+    del ro
     r2 = lambda v: round(v, 1) if v else None
     out = {"target": RMSE_TARGET, "on": "clamp(sqrt(mean), 0, 1), all pixels and channels",
            "against": f"plain GPU render, {ref_spp} spp", "reference_rmse_floor": round(floor, 5),
@@ -141,9 +178,12 @@ def time_to_rmse(tpt, torch, sc, ref_spp, cpu):
            "guided_s": round(t_guided, 4) if t_guided else None, "guided_spp": r2(n_guided),
            "guided_speedup": round(t_plain / t_guided, 3) if t_plain and t_guided else None,
            "guided_schedule": sch, "guided_mix_weight": round(alpha, 3),
+           "overlap_s": round(t_over, 4) if t_over else None, "overlap_spp": r2(n_over),
+           "overlap_how": "training on a second stream beside the passes; wall clock, evaluation deferred",
            "snapshots": f"doubling spp from {SNAPSHOT_SPP} (guided: after the training passes)",
            "plain_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in plain_pts],
-           "guided_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in guided_pts]}
+           "guided_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in guided_pts],
+           "overlap_curve": [[p[0], round(p[1], 4), round(p[2], 5)] for p in over_pts]}
     if cpu is not None:
         # the reference's own render with its progressive snapshots: each
         # snapshot's RMSE, at the time the render took for its samples

@@ -330,3 +330,43 @@ def test_guided_keeps_the_expectation_at_full_size(tpt, device):
     assert abs(dm) <= 4 * se
     assert frac < 0.015
 
+
+
+def test_overlapped_training_keeps_the_expectation(tpt, device):
+    """ProgressiveRender(overlap=True): the Adam steps run on a second stream
+    while the next passes render with whatever weights the guide holds when
+    each block stages them.  The rows handed to training are the recorded
+    ones, the guide trains (the parameters move), and the image keeps the
+    plain estimator's expectation: 160x160 Cornell, 3 training passes of
+    4 spp, then 500 more guided spp, against 512 plain spp on other streams."""
+    import torch
+
+    sc = tpt.Scene.named("cornell555")
+    w = 160
+    cam = sc.host.camera
+    g = tpt.Guide(seed=3)
+    p0 = g.get_params().copy()
+    r = tpt.ProgressiveRender(sc, guide=g, seed=81, record_stride=2, iters=64, batch=1 << 16, lr=5e-3,
+                              width=w, height=w, camera=cam, overlap=True)
+    r.add(4, train=True)
+    torch.cuda.synchronize()
+    rows = g.copy_rows()
+    assert rows.shape[0] > 0 and torch.equal(rows, g.batch_rows())
+    for _ in range(2):
+        r.add(4, train=True)
+    r.add(500)
+    r.finish()
+    torch.cuda.synchronize()
+    assert r.spp == 512
+    assert np.abs(g.get_params() - p0).max() > 1e-3
+    s_g, q_g = (x.cpu().numpy().astype(np.float64) for x in (r.sum, r.sumsq))
+    s_p, q_p, _ = sc.render(width=w, height=w, spp=512, seed=82, camera=cam)
+    s_p, q_p = (x.cpu().numpy().astype(np.float64) for x in (s_p, q_p))
+    spp = 512
+    frac, _ = zfrac(s_g, q_g, s_p, q_p, spp)
+    n = w * w * 3 * spp
+    se = np.sqrt(q_g.mean() / spp / n + q_p.mean() / spp / n)
+    dm = (s_g.mean() - s_p.mean()) / spp
+    print(f"overlap: |z|>3 in {frac:.4f}; image mean diff {dm:.6f} ({dm / se:.2f} SE)")
+    assert abs(dm) <= 4 * se
+    assert frac < 0.015

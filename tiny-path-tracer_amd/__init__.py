@@ -432,15 +432,34 @@ class Guide:
                                         torch.cuda.current_stream().cuda_stream), "tpt_guide_gradient")
         return grad, loss.value
 
-    def train(self, rows=None, n_batch=1 << 18, iters=1, lr=1e-3):
+    def train(self, rows=None, n_batch=1 << 18, iters=1, lr=1e-3, wait=True):
         """Adam steps on `rows` ((n, 16) device tensor) or, if None, on the rows
-        recorded by the last guided render.  Returns the last batch loss."""
+        recorded by the last guided render, on the current stream.  Returns the
+        last batch loss; wait=False queues the steps and returns None at once."""
         torch = _torch()
         loss = C.c_float()
         ptr, n = (rows.data_ptr(), rows.shape[0]) if rows is not None else (None, 0)
-        _check(lib().tpt_guide_train(self.h, ptr, n, n_batch, iters, lr, C.byref(loss),
+        _check(lib().tpt_guide_train(self.h, ptr, n, n_batch, iters, lr, C.byref(loss) if wait else None,
                                      torch.cuda.current_stream().cuda_stream), "tpt_guide_train")
-        return loss.value
+        return loss.value if wait else None
+
+    def copy_rows(self, out=None):
+        """The rows recorded by the last guided render, copied on the current
+        stream into `out` (reused when large enough) or a new tensor."""
+        torch = _torch()
+        ptr = C.c_void_p()
+        n = C.c_uint64()
+        _check(lib().tpt_guide_batch(self.h, C.byref(ptr), C.byref(n)), "tpt_guide_batch")
+        if out is None or out.shape[0] < n.value:
+            out = torch.empty((n.value, ROW), dtype=torch.float32, device="cuda")
+        if n.value:
+            hip = C.CDLL("libamdhip64.so")
+            hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+            rc = hip.hipMemcpyAsync(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3,
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream))
+            if rc != 0:
+                raise TptError(f"hipMemcpyAsync failed ({rc})")
+        return out[:n.value]
 
     def batch_rows(self):
         """Copy of the rows recorded by the last guided render, (n, 16) float32 on the device."""
@@ -468,10 +487,18 @@ class ProgressiveRender:
     records training rows from every record_stride-th path and is followed by
     `iters` Adam steps of `batch` rows on them (the online training loop).
     Every sample is kept, including those of the early, barely trained passes:
-    each is an unbiased sample of the same integral."""
+    each is an unbiased sample of the same integral.
+
+    overlap=True trains on a second stream while the next pass renders: the
+    recorded rows are copied there, the Adam steps queued behind the copy, and
+    the render continues at once with whatever weights the guide holds when
+    each block starts (a block stages them once, so its sampling and its
+    densities use the same network, and the estimate stays unbiased).  The next
+    recording pass waits for the copy before it overwrites the rows.  Results
+    then depend on timing."""
 
     def __init__(self, scene, guide=None, seed=1, record_stride=8, iters=16, batch=1 << 18, lr=1e-3,
-                 guide_vertices=2, width=None, height=None, camera=None):
+                 guide_vertices=2, width=None, height=None, camera=None, overlap=False):
         torch = _torch()
         info = scene.host.info
         self.scene, self.guide, self.seed = scene, guide, seed
@@ -484,21 +511,41 @@ class ProgressiveRender:
         self.sumsq = torch.zeros(shape, dtype=torch.float32, device="cuda")
         self.spp = 0
         self.train_loss = None
+        self.overlap = overlap
+        if overlap:
+            self._stream = torch.cuda.Stream()
+            self._copied = None
+            self._rows = None
 
     def add(self, spp, train=False):
         """Renders spp more samples per pixel (train: record rows and train the
         guide afterwards).  Returns the render stats."""
+        torch = _torch()
         if train and self.guide is None:
             raise TptError("training needs a Guide")
+        if self.overlap and train and self._copied is not None:
+            torch.cuda.current_stream().wait_event(self._copied)  # the last rows are out of the batch
         _, _, st = self.scene.render(width=self.width, height=self.height, spp=spp, seed=self.seed,
                                      sample_offset=self.spp, sum_buf=self.sum, sumsq_buf=self.sumsq,
                                      camera=self.camera, guide=self.guide,
                                      guiding=self.record_stride if train else 0,
                                      guide_vertices=self.guide_vertices)
         self.spp += spp
-        if train:
+        if train and self.overlap:
+            self._stream.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(self._stream):
+                self._rows = self.guide.copy_rows(self._rows)
+                self._copied = torch.cuda.Event()
+                self._copied.record()
+                self.guide.train(self._rows, n_batch=self.batch, iters=self.iters, lr=self.lr, wait=False)
+        elif train:
             self.train_loss = self.guide.train(None, n_batch=self.batch, iters=self.iters, lr=self.lr)
         return st
+
+    def finish(self):
+        """Waits for training queued by overlap mode."""
+        if self.overlap:
+            self._stream.synchronize()
 
     def mean(self):
         return self.sum / max(self.spp, 1)
