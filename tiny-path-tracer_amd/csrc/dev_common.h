@@ -50,6 +50,7 @@ struct scene_view {
   const tpt_motion *motions;
   const uint32_t *seg_xf;  // per segment: xform | flip<<31 (TPT_SEGMENT_PLAIN), or null: none wrapped
   uint32_t n_slots, n_segments, n_lights, light_mode;
+  float walk_ulp;  // 2^-22 x the largest coordinate magnitude of the scene (sphere-only walk bound)
 };
 
 // --------------------------------------------------------------- Philox ---

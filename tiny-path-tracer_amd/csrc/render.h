@@ -83,7 +83,7 @@ struct job_dev {
 // box (position encoding of the guide).
 struct scene_dev {
   uint32_t seg_begin, n_seg, light_begin, n_lights, light_mode, pad[3];
-  float bounds[6], pad2[2];
+  float bounds[6], walk_ulp, pad2;  // walk_ulp: scene_view::walk_ulp of this scene
 };
 
 // Pass table: items [item_begin, next entry's item_begin) of the pass are
@@ -164,6 +164,7 @@ __device__ __forceinline__ scene_view view_of(const scene_view &S, const render_
   v.lights = S.lights + sc.light_begin;
   v.n_lights = sc.n_lights;
   v.light_mode = sc.light_mode;
+  v.walk_ulp = sc.walk_ulp;
   return v;
 }
 

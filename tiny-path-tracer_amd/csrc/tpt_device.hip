@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -20,7 +21,8 @@ using namespace tptd;
 namespace {
 
 // -------------------------------------------------------- closest hit API ---
-template <bool SWEEP, bool MEDIA>
+// SPH: a sphere-only scene takes the render's sphere-only walk.
+template <bool SWEEP, bool MEDIA, bool SPH = false>
 __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_slot *__restrict__ slots,
                                                        const tpt_entry *__restrict__ entries,
                                                        const float *__restrict__ rays,
@@ -42,7 +44,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(scene_view S, const tpt_s
   }
   hit_t h;
   uint32_t nodes = 0;
-  const bool got = trace_world<SWEEP, false, MEDIA>(S, o, d, rc, tmin, tmax, &h, &nodes, live);
+  const bool got = trace_world<SWEEP, SPH, MEDIA>(S, o, d, rc, tmin, tmax, &h, &nodes, live);
   if (!live) return;
   if (got) {
     leaf[i] = static_cast<int32_t>(leaf_id(S, h));
@@ -156,6 +158,7 @@ struct tpt_scene {
     v.n_segments = scn[k].n_seg;
     v.n_lights = scn[k].n_lights;
     v.light_mode = scn[k].light_mode;
+    v.walk_ulp = scn[k].walk_ulp;
     return v;
   }
   // The pooled arrays with every segment and light (multi-job launches).
@@ -456,6 +459,25 @@ int create_pool(const tpt_flat_scene *scenes, uint32_t n, tpt_scene **out) {
     d.n_lights = f.n_lights;
     d.light_mode = f.n_lights ? f.light_mode : 0;
     for (int q = 0; q < 6; ++q) d.bounds[q] = f.bounds[q];
+    // Sphere-only walk (traverse.h, sphere_walk_bound): the scene's largest
+    // coordinate magnitude, and the boxes holding a sphere of radius above
+    // kWalkBigRadius flagged in the device copy of their skip word.
+    if (sph[k]) {
+      float mag = 0.0f;
+      bool finite = true;
+      std::vector<uint32_t> big(f.n_slots + 1, 0u);
+      for (uint32_t i = 0; i < f.n_slots; ++i) {
+        const tpt_slot &s = f.slots[i];
+        const bool box = (s.b >> 28) == TPT_SLOT_BOX;
+        for (int q = 0; q < (box ? 6 : 4); ++q) finite = finite && std::fabs(s.f[q]) < INFINITY;
+        for (int q = 0; q < (box ? 6 : 3); ++q) mag = std::fmax(mag, std::fabs(s.f[q]) + (box ? 0.0f : std::fabs(s.f[3])));
+        big[i + 1] = big[i] + (!box && !(std::fabs(s.f[3]) <= kWalkBigRadius) ? 1u : 0u);
+      }
+      // a non-finite coordinate leaves every box at t_max (an infinite reach)
+      d.walk_ulp = finite ? std::ldexp(mag, -22) : INFINITY;
+      for (uint32_t i = 0; i < f.n_slots; ++i)
+        if ((f.slots[i].b >> 28) == TPT_SLOT_BOX && big[f.slots[i].a] != big[i]) slots[sb + i].a |= kBoxUnbounded;
+    }
     motion[k] = f.n_motions > 0;
     media[k] = 0;
     for (uint32_t i = 0; i < f.n_slots; ++i) media[k] |= (f.slots[i].b >> 28) == TPT_SLOT_MEDIUM;
@@ -605,6 +627,10 @@ int tpt_trace_closest_timed(tpt_scene *s, const float *d_rays, const float *d_ti
                        d_leaf, d_t, d_rec, d_mat);
   else if (s->media[0])
     hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
+                       d_leaf, d_t, d_rec, d_mat);
+  else if (s->sph[0])
+    hipLaunchKernelGGL((k_trace_closest<false, false, true>), dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), S, S.slots, S.entries, d_rays, d_time, n, t_min, t_max,
                        d_leaf, d_t, d_rec, d_mat);
   else

@@ -35,6 +35,11 @@ namespace tptd {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr float kFltMaxT = 3.402823466e38f;  // std::numeric_limits<float>::max()
+// Sphere-only walk: a box whose subtree holds a sphere of radius above
+// kWalkBigRadius has kBoxUnbounded set in its skip word (device copy only;
+// skip links are below 2^28) and is always tested against t_max.
+constexpr float kWalkBigRadius = 2.0f;
+constexpr uint32_t kBoxUnbounded = 0x80000000u;
 
 struct hit_t {
   float t;
@@ -389,8 +394,36 @@ __device__ __forceinline__ bool segment_exact(const scene_view &S, uint32_t star
   return have;
 }
 
+// Box bound of the sphere-only walk once a hit at t is held: no sphere in a
+// box entered beyond it has a computed t at or below t.  In units of t (the
+// ray's |d| as the unit of length):
+//  * a sphere's exact entry lies in its exact box [c - r, c + r]; the box
+//    holds c -+ r rounded to float (within 2^-24 M of exact, M the scene's
+//    largest coordinate magnitude; bvh boxes are exact unions), so the
+//    computed box entry exceeds the sphere's exact entry by at most
+//    2^-24 M max|1/d_i| plus two roundings of the slab arithmetic;
+//  * sphere_t's float coefficients a, b, c move a root by at most about
+//    sqrt(10 eps) D / (2|d|) ~ 4e-4 (|t| + r/|d|) (D = |o - c|; worst at
+//    tangency, where the rounding of b^2 - 4ac passes through the square root).
+// The bound adds 2^-6 (|t| + |t_min|), 2^-6 kWalkBigRadius / |d| and
+// 2^-22 M max|1/d_i| (reach, per ray): each at least 4x its term, 39x for the
+// root error.  Spheres larger than kWalkBigRadius are never bounded (their
+// boxes are flagged).  So a skipped box holds only spheres with t strictly
+// above the held one, and skipping it changes neither the winner nor a tie.
+// A non-finite reach (a zero direction component, an infinite coordinate)
+// leaves t_max.
+__device__ __forceinline__ float sphere_walk_bound(float t, float tmin, float tmax, float reach) {
+  return fminf(tmax, t + (fabsf(t) + fabsf(tmin)) * 0.015625f + reach);
+}
+
 // Fast pre-order walk of one segment (t_min, t_max not NaN).  *saw_nan is set
 // when a leaf returned a NaN t (the caller then uses segment_exact).
+//
+// SPH (every leaf a sphere, so no NaN t): boxes are tested against
+// sphere_walk_bound of the hit held so far instead of t_max.  The reference
+// tests every box the segment crosses (bvh_node::hit passes the same t_max to
+// both children); the skipped boxes hold no sphere that could win, so the
+// result is the reference's bit for bit.  Leaves keep the caller's bounds.
 template <bool SPH = false, bool MEDIA = false>
 __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start, uint32_t end, v3 o, v3 d, v3 inv,
                                              const ray_ctx &rc, float tmin, float tmax, hit_t *out, bool *saw_nan,
@@ -399,6 +432,11 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t i = start;
   uint32_t visited = 0;
+  float bound = tmax;  // box bound (SPH: shrinks with the held hit)
+  float reach = 0.0f;
+  if (SPH)
+    reach = 0.015625f * kWalkBigRadius * rsqrtf(d.x * d.x + d.y * d.y + d.z * d.z) +
+            S.walk_ulp * fmaxf(fmaxf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
   while (i < end) {
     const slot_t s = load_slot(&S.slots[i]);
     const uint32_t kind = s.b >> 28;
@@ -406,7 +444,8 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
     uint32_t ni = i + 1;
     if (kind == TPT_SLOT_BOX) {
       if (oc) ++oc->box;
-      if (!slab_nn(s, o, inv, tmin, tmax)) ni = s.a;
+      const bool unbounded = SPH && (s.a & kBoxUnbounded);
+      if (!slab_nn(s, o, inv, tmin, unbounded ? tmax : bound)) ni = s.a & TPT_PAYLOAD_MASK;
     } else {
       float t;
       uint32_t entry, mat;
@@ -415,6 +454,7 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
         if (!have || t <= best.t) {
           best = hit_t{t, i, entry, mat};
           have = true;
+          if (SPH) bound = sphere_walk_bound(t, tmin, tmax, reach);
         }
       }
     }
@@ -459,7 +499,7 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
       ++visited;
       if (kind == TPT_SLOT_BOX) {
         if (oc) ++oc->box;
-        next = slab_nn(s, o, inv, tmin, tmax) ? i + 1 : s.a;
+        next = slab_nn(s, o, inv, tmin, tmax) ? i + 1 : (s.a & TPT_PAYLOAD_MASK);
       } else {
         float t;
         uint32_t entry, mat;
