@@ -56,6 +56,9 @@ struct scene_view {
 // --------------------------------------------------------------- Philox ---
 // Philox4x32-10 (Salmon et al., SC'11).  Render streams: draw k of bounce b of
 // sample s of pixel p is word k%4 of philox(ctr = (p, s, b, k/4), key = seed).
+// Each round's two 32x32 -> 64-bit products are written as 64-bit multiplies
+// of zero-extended words, one v_mad_u64_u32 each instead of a separate
+// v_mul_lo_u32 and v_mul_hi_u32 (same bits).
 __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                         uint32_t k1) {
 #pragma unroll
@@ -64,8 +67,10 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+    const uint32_t lo0 = static_cast<uint32_t>(p0), hi0 = static_cast<uint32_t>(p0 >> 32);
+    const uint32_t lo1 = static_cast<uint32_t>(p1), hi1 = static_cast<uint32_t>(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
     c1 = lo1;
