@@ -111,6 +111,18 @@ class RenderStats(C.Structure):
 
 
 _lib = None
+_hip_lib = None
+
+
+def _hip():
+    """The HIP runtime (device-to-device copies of guide rows), loaded once."""
+    global _hip_lib
+    if _hip_lib is None:
+        h = C.CDLL("libamdhip64.so")
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        _hip_lib = h
+    return _hip_lib
 
 
 def lib():
@@ -435,7 +447,9 @@ class Guide:
     def train(self, rows=None, n_batch=1 << 18, iters=1, lr=1e-3, wait=True):
         """Adam steps on `rows` ((n, 16) device tensor) or, if None, on the rows
         recorded by the last guided render, on the current stream.  Returns the
-        last batch loss; wait=False queues the steps and returns None at once."""
+        last batch loss; wait=False queues the steps and returns None at once.
+        Steps on one guide share its gradient buffers: queue them on one
+        stream at a time."""
         torch = _torch()
         loss = C.c_float()
         ptr, n = (rows.data_ptr(), rows.shape[0]) if rows is not None else (None, 0)
@@ -453,9 +467,7 @@ class Guide:
         if out is None or out.shape[0] < n.value:
             out = torch.empty((n.value, ROW), dtype=torch.float32, device="cuda")
         if n.value:
-            hip = C.CDLL("libamdhip64.so")
-            hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
-            rc = hip.hipMemcpyAsync(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3,
+            rc = _hip().hipMemcpyAsync(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3,
                                     C.c_void_p(torch.cuda.current_stream().cuda_stream))
             if rc != 0:
                 raise TptError(f"hipMemcpyAsync failed ({rc})")
@@ -470,9 +482,7 @@ class Guide:
         out = torch.empty((n.value, ROW), dtype=torch.float32, device="cuda")
         if n.value:
             torch.cuda.synchronize()
-            hip = C.CDLL("libamdhip64.so")
-            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-            rc = hip.hipMemcpy(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3)  # device to device
+            rc = _hip().hipMemcpy(C.c_void_p(out.data_ptr()), ptr, n.value * ROW * 4, 3)  # device to device
             if rc != 0:
                 raise TptError(f"hipMemcpy failed ({rc})")
         return out
