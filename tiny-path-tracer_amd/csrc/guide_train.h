@@ -2,9 +2,8 @@
 // forward/backward training step (BF16 MFMA, FP32 accumulation) and the
 // gradient reduction + Adam update.
 //
-// Training step (k_guide_train).  Each wave works alone on 16 samples at a
-// time (lane l: sample l%16, lane group g = l/16), with no LDS traffic and no
-// block barrier inside its loop:
+// Training step (k_guide_train).  Each wave works on 16 samples at a time
+// (lane l: sample l%16, lane group g = l/16):
 //   1. forward in the inference layout (16x16x32, permuted K order, guide.h):
 //      the D tiles, "sample-major" (a lane holds neurons 16mt + 4g + r of one
 //      sample), feed the next layer directly.  The output rows are permuted so
@@ -19,10 +18,16 @@
 //      gives exactly that layout from the same registers: activations from
 //      (activation, W) products, deltas from (delta, Wᵀ) products, and the
 //      inputs and output deltas from products with identity tiles;
-//   5. dW += δ·hᵀ over the 16 samples with v_mfma_f32_16x16x16_bf16, the 56
-//      tiles accumulated in registers for the whole launch.
-// The four waves of a block then sum their tiles through LDS into one block
-// partial; k_guide_reduce adds the partials into an accumulator and
+//   5. dW += δ·hᵀ with v_mfma_f32_16x16x16_bf16.  The 56 dW tiles are split
+//      over the block's 8 waves (7 each, at most 2 per layer): each wave
+//      publishes a layer's neuron-major deltas and activations in LDS, and
+//      after one block barrier every wave adds the products of all 8 waves'
+//      samples into the tiles it owns.  28 accumulator registers per wave
+//      instead of 224 let two waves share each SIMD (512-thread blocks, one
+//      66 KB copy of the weight tiles per CU); the exchange is double
+//      buffered, so one barrier per layer suffices.
+// Each wave then writes its own tiles into the block partial (every tile has
+// one owner); k_guide_reduce adds the partials into an accumulator and
 // k_guide_adam turns it into the gradient or the Adam update (and clears it).
 #ifndef TPT_CSRC_GUIDE_TRAIN_H
 #define TPT_CSRC_GUIDE_TRAIN_H
@@ -199,21 +204,40 @@ __device__ __forceinline__ void lobe_grad(const lobe &L, v3 om, float f, float g
   o5[4] = f * L.w * (L.d - g);
 }
 
+// Block of the training step: 8 waves, 128 samples per iteration.
+constexpr int GT_WAVES = 8, GT_THREADS = 64 * GT_WAVES, GT_ROWS = 16 * GT_WAVES;
+// dW exchange: 2 buffers x 8 waves x 8 operands (dn 0..3, hn 0..3) x 64 lanes
+// of bf16x4, operand-major so that a wave's 64 lanes touch consecutive words.
+constexpr int GT_XCH = 2 * GT_WAVES * 8 * 64;
+
 constexpr size_t k_guide_train_lds() {
-  return GR_TILES * 1024 + G_BIAS * 4 + 3 * 14 * 1024 + (G_BIAS + 4) * 4;
+  return GR_TILES * 1024 + G_BIAS * 4 + GT_XCH * 8 + (G_BIAS + 4) * 4;
+}
+
+// dW tile owned by slot k (0..7) of wave w: layer L = 3 - k/2, index
+// q = base + 8 (k%2) within the layer (row-major over (mo, ni)), base = w
+// except for W1 (L = 0), where (w + 4) % 8 evens out the 12-tile layers.
+// False if the slot holds no tile (a 12-tile layer's second slot).
+__device__ __forceinline__ bool owned_tile(int w, int k, int *L, int *mo, int *ni) {
+  const int l = 3 - k / 2;
+  const int nt = (l == 3 || l == 0) ? 12 : 16, nc = l == 0 ? 3 : 4;
+  const int q = (l == 0 ? (w + 4) % 8 : w) + 8 * (k % 2);
+  *L = l, *mo = q / nc, *ni = q % nc;
+  return q < nt;
 }
 
 // Fused forward/backward over rows [0, n) (row k read at (row0 + k) % n_total).
-// 256 threads per block; each wave takes groups of 16 rows.  Writes this
-// block's partial gradient (natural parameter layout) to partial.
-__global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict__ rtiles, const float *__restrict__ params,
-                                                     const float *__restrict__ rows, uint64_t n, uint64_t row0,
-                                                     uint64_t n_total, float *__restrict__ partial) {
+// GT_THREADS threads per block; each wave takes groups of 16 rows, the block
+// 128 per iteration.  Writes this block's partial gradient (natural parameter
+// layout) to partial.
+__global__ void __launch_bounds__(GT_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+k_guide_train(const uint32_t *__restrict__ rtiles, const float *__restrict__ params, const float *__restrict__ rows,
+              uint64_t n, uint64_t row0, uint64_t n_total, float *__restrict__ partial) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t *wt = reinterpret_cast<uint32_t *>(lds);
   float *bias = reinterpret_cast<float *>(wt + GR_TILES * 256);  // b1 b2 b3 natural, b4 in lobe order
-  f32x4 *xch = reinterpret_cast<f32x4 *>(bias + G_BIAS);         // 3 waves x 14 tiles x 64 lanes
-  float *sums = reinterpret_cast<float *>(xch + 3 * 14 * 64);    // biases, mix-logit gradient, loss
+  bf16x4 *xch = reinterpret_cast<bf16x4 *>(bias + G_BIAS);       // dW exchange (GT_XCH entries)
+  float *sums = reinterpret_cast<float *>(xch + GT_XCH);         // biases, mix-logit gradient, loss
   guide_stage_weights(wt, rtiles, GR_TILES * 256);
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) {
     float b = 0.0f;
@@ -227,16 +251,27 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
   const int l = __lane_id(), wave = threadIdx.x / 64, g = l >> 4, j = l & 15;
   const float inv_n = 1.0f / static_cast<float>(n);
   const float alpha = 1.0f / (1.0f + __expf(-params[G_NPARAM_PAD]));
-  // dW tiles: W4 3x4 (0..11), W3 4x4 (12..27), W2 4x4 (28..43), W1 4x3 (44..55)
-  f32x4 dw[56];
+  // the dW tiles this wave owns (owned_tile), accumulated over the launch
+  f32x4 dw[8];
 #pragma unroll
-  for (int t = 0; t < 56; ++t) dw[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 8; ++k) dw[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  int own_mo[8], own_ni[8];
+  bool own[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int L;
+    own[k] = owned_tile(wave, k, &L, &own_mo[k], &own_ni[k]);
+  }
+  int xbuf = 0;
   float db[15];  // per lane: neuron 16 t + j of b1 (0..3), b2 (4..7), b3 (8..11), b4 lobe-order rows (12..14)
 #pragma unroll
   for (int k = 0; k < 15; ++k) db[k] = 0.0f;
   float dalpha = 0.0f, loss = 0.0f;
-  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * 4;
-  for (uint64_t base = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * 16; base < n; base += nwaves * 16) {
+  // every wave of a block runs the same iterations (the dW exchange has block
+  // barriers); a wave past the end works on zero-weight rows
+  for (uint64_t bbase = static_cast<uint64_t>(blockIdx.x) * GT_ROWS; bbase < n;
+       bbase += static_cast<uint64_t>(gridDim.x) * GT_ROWS) {
+    const uint64_t base = bbase + 16 * wave;
     // an opaque zero offset keeps the weight-tile loads inside the loop (hoisted
     // out of it they would hold the whole tile set in registers)
     int zoff = 0;
@@ -382,15 +417,27 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
           dn[t] = bf16x4{0, 0, 0, 0};
         }
       }
-      const int base_t = L == 3 ? 0 : (L == 2 ? 12 : (L == 1 ? 28 : 44));
-      const int nc = L == 0 ? 3 : 4;
+      // publish δ_L and h_L (neuron-major, this wave's 16 samples), then add
+      // every wave's products into the owned tiles of layer L
+      bf16x4 *xw = xch + (xbuf * GT_WAVES + wave) * 8 * 64 + l;
 #pragma unroll
-      for (int mo = 0; mo < 4; ++mo)
+      for (int t = 0; t < 4; ++t) {
+        xw[64 * t] = dn[t];
+        xw[64 * (4 + t)] = hn[t];
+      }
+      __syncthreads();
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-          if (mo < nt && ni < nc)
-            dw[base_t + nc * mo + ni] =
-                __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(dn[mo], hn[ni], dw[base_t + nc * mo + ni], 0, 0, 0);
+      for (int e = 0; e < 2; ++e) {
+        const int k = 2 * (3 - L) + e;
+        if (own[k]) {
+          const bf16x4 *xa_ = xch + xbuf * GT_WAVES * 8 * 64 + 64 * own_mo[k] + l;
+          const bf16x4 *xb_ = xch + xbuf * GT_WAVES * 8 * 64 + 64 * (4 + own_ni[k]) + l;
+#pragma unroll
+          for (int src = 0; src < GT_WAVES; ++src)
+            dw[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa_[src * 8 * 64], xb_[src * 8 * 64], dw[k], 0, 0, 0);
+        }
+      }
+      xbuf ^= 1;
       // δ_L sample-major, for the next iteration: (W_{L+1}ᵀ δ_{L+1}) ⊙ ReLU'
       if (L == 2 || L == 1) {
         const uint32_t sm = L == 2 ? m3 : m2;
@@ -408,22 +455,6 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
       }
       dmask = nm;
     }
-  }
-  // ---- block reduction: waves 1..3 hand their tiles to wave 0, 14 at a time
-#pragma unroll
-  for (int chunk = 0; chunk < 4; ++chunk) {
-    if (wave > 0) {
-#pragma unroll
-      for (int t = 0; t < 14; ++t) xch[((wave - 1) * 14 + t) * 64 + l] = dw[14 * chunk + t];
-    }
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-      for (int t = 0; t < 14; ++t)
-#pragma unroll
-        for (int w = 0; w < 3; ++w) dw[14 * chunk + t] += xch[(w * 14 + t) * 64 + l];
-    }
-    __syncthreads();
   }
   // biases, mix logit, loss: over the lane groups, then the waves
 #pragma unroll
@@ -446,23 +477,20 @@ __global__ void __launch_bounds__(256) k_guide_train(const uint32_t *__restrict_
   }
   __syncthreads();
   float *P = partial + static_cast<size_t>(blockIdx.x) * G_PARTIAL;
-  if (wave == 0) {
-    // dW tile (L, mo, ni): lane (g, j) register r = dW[16 mo + 4 g + r][column 16 ni + j]
+  // this wave's tiles: tile (L, mo, ni), lane (g, j), register r holds
+  // dW[16 mo + 4 g + r][column 16 ni + j]
 #pragma unroll
-    for (int t = 0; t < 56; ++t) {
-      const int L = t < 12 ? 3 : (t < 28 ? 2 : (t < 44 ? 1 : 0));
-      const int k = t < 12 ? t : (t < 28 ? t - 12 : (t < 44 ? t - 28 : t - 44));
-      const int nc = L == 0 ? 3 : 4;
-      const int mo = k / nc, ni = k % nc;
-      int off, rws, cols;
-      layer_geom(L, &off, &rws, &cols);
-      const int col = L == 0 ? xslot_feature(ni, j) : 16 * ni + j;  // identity column 16 g + t = input slot (g, t)
+  for (int k = 0; k < 8; ++k) {
+    if (!own[k]) continue;
+    const int L = 3 - k / 2, mo = own_mo[k], ni = own_ni[k];
+    int off, rws, cols;
+    layer_geom(L, &off, &rws, &cols);
+    const int col = L == 0 ? xslot_feature(ni, j) : 16 * ni + j;  // identity column 16 g + t = input slot (g, t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int rr = 16 * mo + 4 * g + r;
-        const int row_ = L == 3 ? out_logical(rr) : rr;
-        if (row_ >= 0 && col >= 0) P[off + row_ * cols + col] = dw[t][r];
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int rr = 16 * mo + 4 * g + r;
+      const int row_ = L == 3 ? out_logical(rr) : rr;
+      if (row_ >= 0 && col >= 0) P[off + row_ * cols + col] = dw[k][r];
     }
   }
   for (int i = threadIdx.x; i < G_BIAS; i += blockDim.x) {

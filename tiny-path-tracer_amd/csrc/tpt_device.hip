@@ -570,9 +570,10 @@ int guide_pack(tpt_guide *g, hipStream_t stream) {
 // n_total), partials reduced by k_guide_reduce (mode 0: gradient, 1: Adam).
 int guide_step(tpt_guide *g, const float *rows, uint64_t n, uint64_t row0, uint64_t n_total, int mode, float lr,
                float *grad, float *loss_dev, hipStream_t stream) {
-  const int blocks = static_cast<int>(std::min<uint64_t>((n + 63) / 64, static_cast<uint64_t>(g->train_blocks)));
+  const int blocks =
+      static_cast<int>(std::min<uint64_t>((n + GT_ROWS - 1) / GT_ROWS, static_cast<uint64_t>(g->train_blocks)));
   const size_t lds = k_guide_train_lds();
-  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), lds, stream, g->rtiles, g->params, rows, n, row0,
+  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(GT_THREADS), lds, stream, g->rtiles, g->params, rows, n, row0,
                      n_total, g->partial);
   HIP_TRY(hipGetLastError());
   if (mode == 1) ++g->step;
