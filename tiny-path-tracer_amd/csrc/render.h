@@ -477,8 +477,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
                            ? make_ray_ctx(P, item_in_entry(P, ent, __float_as_uint(a.w)), depth)
                            : ray_ctx{0.0f, 0u, 0u, 0u, 0u, 0u};
     op_counts oc;
-    const bool got = trace_world<SWEEP, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax,
-                                                    &h, &st[ST_NODES], valid, COUNT ? &oc : nullptr);
+    bool got;
+    if (SPH) {
+      // A wave of camera rays of one job walks nearly the same slots (about
+      // the longest lane's walk in all): it sweeps the tree together, with
+      // scalar slot loads and no divergence on the slot kind.
+      const unsigned long long vm = __ballot(valid);
+      const int first = vm ? __ffsll(static_cast<long long>(vm)) - 1 : 0;
+      const uint32_t job_u = __builtin_amdgcn_readlane(job, first);
+      const bool same = !valid || (depth == 0 && job == job_u);
+      if (vm != 0ull && __ballot(!same) == 0ull)
+        got = trace_world<true, SPH, false>(view_of<MULTI>(S, P, job_u), v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc,
+                                            0.001f, kFltMax, &h, &st[ST_NODES], valid, COUNT ? &oc : nullptr);
+      else
+        got = trace_world<false, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax, &h,
+                                             &st[ST_NODES], valid, COUNT ? &oc : nullptr);
+    } else {
+      got = trace_world<SWEEP, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax, &h,
+                                           &st[ST_NODES], valid, COUNT ? &oc : nullptr);
+    }
     if (COUNT) {
       st[ST_BOX] += oc.box;
       st[ST_SPHERE] += oc.sphere;

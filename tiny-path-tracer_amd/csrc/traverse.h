@@ -482,7 +482,11 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // in the same order (same results as segment_fast), while the slot index is
 // wave-uniform: slot data comes through scalar loads and the branch on the
 // slot kind does not diverge.  Lanes with `live` false take no part.
-// Must be called by all lanes of the wave together.
+// Must be called by all lanes of the wave together.  SPH: a sphere-only
+// scene, each lane's boxes bounded by its own held hit as in segment_fast
+// (the sweep then serves large trees for rays whose walks mostly coincide,
+// such as one wave's camera rays).
+template <bool SPH = false>
 __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t start, uint32_t end, bool live, v3 o, v3 d,
                                               v3 inv, const ray_ctx &rc, float tmin, float tmax, hit_t *out,
                                               bool *saw_nan, uint32_t *nodes, op_counts *oc = nullptr) {
@@ -490,6 +494,11 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
   hit_t best{0.0f, kNone, kNone, 0};
   uint32_t next = live ? start : end;
   uint32_t visited = 0;
+  float bound = tmax;
+  float reach = 0.0f;
+  if (SPH)
+    reach = 0.015625f * kWalkBigRadius * rsqrtf(d.x * d.x + d.y * d.y + d.z * d.z) +
+            S.walk_ulp * fmaxf(fmaxf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
   uint32_t i = __builtin_amdgcn_readfirstlane(wave_min(next));
   while (i < end) {
     const slot_t s = load_slot(&S.slots[i]);
@@ -499,15 +508,17 @@ __device__ __forceinline__ bool segment_sweep(const scene_view &S, uint32_t star
       ++visited;
       if (kind == TPT_SLOT_BOX) {
         if (oc) ++oc->box;
-        next = slab_nn(s, o, inv, tmin, tmax) ? i + 1 : (s.a & TPT_PAYLOAD_MASK);
+        const bool unbounded = !SPH || (s.a & kBoxUnbounded);
+        next = slab_nn(s, o, inv, tmin, unbounded ? tmax : bound) ? i + 1 : (s.a & TPT_PAYLOAD_MASK);
       } else {
         float t;
         uint32_t entry, mat;
-        if (leaf_t(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat, oc)) {
+        if (leaf_t<SPH>(S, s, kind, o, d, rc, tmin, tmax, &t, &entry, &mat, oc)) {
           if (t != t) *saw_nan = true;
           if (!have || t <= best.t) {
             best = hit_t{t, i, entry, mat};
             have = true;
+            if (SPH) bound = sphere_walk_bound(t, tmin, tmax, reach);
           }
         }
         next = i + 1;
@@ -554,7 +565,7 @@ __device__ __forceinline__ bool trace_world(const scene_view &S, v3 o, v3 d, con
     const bool bounded = tmin == tmin && cur == cur;
     bool got = false;
     if (SWEEP) {
-      got = segment_sweep(S, start, end, live && bounded, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes, oc);
+      got = segment_sweep<SPH>(S, start, end, live && bounded, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes, oc);
     } else {
       if (!live) continue;
       if (bounded) got = segment_fast<SPH, MEDIA>(S, start, end, o, d, inv, rc, tmin, cur, &h, &saw_nan, nodes, oc);
