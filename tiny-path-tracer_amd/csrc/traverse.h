@@ -466,15 +466,9 @@ __device__ __forceinline__ bool segment_fast(const scene_view &S, uint32_t start
   return have;
 }
 
-// Smallest value over the active lanes of the wave.
-__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint32_t o = __shfl_xor(v, off);
-    v = o < v ? o : v;
-  }
-  return v;
-}
+// Smallest value over the active lanes of the wave (the device library's
+// wavefront reduction, DPP moves instead of LDS permutes).
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) { return __ockl_wfred_min_u32(v); }
 
 // Wave-uniform sweep of one segment, for small scenes: the wave steps through
 // the slots together, always at the smallest slot some lane still has to
