@@ -178,9 +178,9 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
   const uint32_t total = __popcll(mask);
   uint32_t base = 0;
   if (total) {
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;  // wave-uniform
     if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(counter, total);
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);
   }
   return base + lane_prefix(mask);
 }
@@ -243,10 +243,10 @@ __device__ __forceinline__ uint32_t claim_item(const pass_hdr &H, uint32_t *coun
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= hi)
       continue;
     const uint32_t cnt = __popcll(need);
-    const int leader = __ffsll(static_cast<long long>(need)) - 1;
+    const int leader = __ffsll(static_cast<long long>(need)) - 1;  // wave-uniform
     uint32_t base = 0;
     if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(cur, cnt);
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);
     const uint32_t avail = base < hi ? (hi - base < cnt ? hi - base : cnt) : 0u;
     const bool mine = (need >> __lane_id()) & 1ull;
     const uint32_t rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
