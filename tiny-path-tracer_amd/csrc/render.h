@@ -295,12 +295,18 @@ __device__ __forceinline__ void st_once(float4 *p, float4 v) {
   __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(p));
 }
 
-__device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path &p) {
-  const uint32_t nan_bits = (p.rad.x != p.rad.x ? 1u : 0u) | (p.rad.y != p.rad.y ? 2u : 0u) |
-                            (p.rad.z != p.rad.z ? 4u : 0u);
-  st_once(&q[0][at], make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item)));
-  st_once(&q[1][at], make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth | (p.ent << 16))));
-  st_once(&q[2][at], make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff | (nan_bits << 16))));
+// The record comes through a plain pointer.  Through a reference (known
+// dereferenceable) the function's own simplification, before it is inlined,
+// merges each plane's three field reads into one 16-byte load that straddles
+// two fields of the caller's record; the caller's record then cannot be split
+// into registers, and the compiler keeps it in LDS (10 KB per block) with an
+// LDS round trip per field access.
+__device__ __forceinline__ void store_ray(float4 *const *q, uint32_t at, const path *p) {
+  const uint32_t nan_bits = (p->rad.x != p->rad.x ? 1u : 0u) | (p->rad.y != p->rad.y ? 2u : 0u) |
+                            (p->rad.z != p->rad.z ? 4u : 0u);
+  st_once(&q[0][at], make_float4(p->o.x, p->o.y, p->o.z, __uint_as_float(p->item)));
+  st_once(&q[1][at], make_float4(p->d.x, p->d.y, p->d.z, __uint_as_float(p->depth | (p->ent << 16))));
+  st_once(&q[2][at], make_float4(p->thr.x, p->thr.y, p->thr.z, __uint_as_float(p->ndiff | (nan_bits << 16))));
 }
 
 // ONCE: the entry's last read (non-temporal); k_guide_shade reads its
@@ -417,7 +423,8 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
   }
   for (uint32_t w = gid; w < H.n_first; w += stride) {
     const uint32_t m = w / 64;
-    store_ray(Q.rq[0], (m % NSUB) * Q.region + 64 * (m / NSUB) + w % 64, raygen_item<MULTI>(P, H, w));
+    const path p = raygen_item<MULTI>(P, H, w);
+    store_ray(Q.rq[0], (m % NSUB) * Q.region + 64 * (m / NSUB) + w % 64, &p);
   }
 }
 
@@ -588,18 +595,34 @@ __device__ __forceinline__ void finish_sample(const render_params &P, const rend
 }
 
 // Finished lanes claim the next work item; every continuing path goes to the
-// next ray queue.  All lanes call.
+// next ray queue.  All lanes call.  (The record comes through a plain pointer,
+// for the reason given at store_ray.)
 template <bool MULTI>
 __device__ __forceinline__ void claim_and_append(const render_params &P, const pass_hdr &H, const render_state &Q,
-                                                 uint32_t *next_count, float4 *const *out, uint32_t home, path &p,
+                                                 uint32_t *next_count, float4 *const *out, uint32_t home, path *p,
                                                  bool cont, bool finished) {
   const uint32_t item = claim_item(H, Q.counters, finished);
   if (finished && item < H.n_items) {
-    p = raygen_item<MULTI>(P, H, item);
+    *p = raygen_item<MULTI>(P, H, item);
     cont = true;
   }
   const uint32_t at = home * Q.region + wave_append(next_count, cont);
   if (cont) store_ray(out, at, p);
+}
+
+// The same with the record by reference, for k_shade.  Through the reference
+// the record's three plane reads are merged into 16-byte loads (see
+// store_ray), which keeps the first 40 bytes of k_shade's record (origin,
+// direction, throughput, one radiance channel) in LDS instead of registers.
+// The single-scene SIMPLE variant then fits 64 VGPRs (8 waves per SIMD)
+// without spilling; with the record in registers it spills 6 VGPRs to scratch
+// and measured 1.5-2.6% slower on the Cornell frame (2-3% slower at a 7-wave
+// target, 1.5-3% with the record in an explicit LDS array).
+template <bool MULTI>
+__device__ __forceinline__ void claim_and_append_ref(const render_params &P, const pass_hdr &H, const render_state &Q,
+                                                     uint32_t *next_count, float4 *const *out, uint32_t home, path &p,
+                                                     bool cont, bool finished) {
+  claim_and_append<MULTI>(P, H, Q, next_count, out, home, &p, cont, finished);
 }
 
 // One queued hit (or miss) of class cls with the plain sampler: the
@@ -720,8 +743,8 @@ __device__ __forceinline__ bool shade_hit(const scene_view &S, const scene_view 
 // use the guide) is shaded by k_guide_shade.  SIMPLE: constant textures, at
 // most one rect light (see shade.h) and no moving spheres.  The occupancy targets below make the
 // register allocator fit 96 registers (5 waves per SIMD instead of 4) without
-// spilling, or, for the single-scene SIMPLE variant, 64 (8 waves per SIMD,
-// with ten registers spilled: measured 3% faster than 6 waves without spills).
+// spilling, or, for the single-scene SIMPLE variant, 64 (8 waves per SIMD;
+// 40 bytes of the record are held in LDS, see claim_and_append_ref).
 template <bool GUIDED, bool MULTI, bool SIMPLE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE ? (MULTI ? 5 : 8) : (GUIDED ? 1 : 5)))) k_shade(scene_view S, render_params P, render_state Q, uint32_t parity) {
   const pass_hdr H = *P.hdr;
@@ -760,7 +783,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
         finished = true;
       }
     }
-    claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
+    claim_and_append_ref<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
   }
   block_stats(Q.stats, st, ST_BASIC);
 }
@@ -936,7 +959,7 @@ __global__ void __launch_bounds__(guide_shade_threads(SIMPLE)) __attribute__((am
         finished = true;
       }
     }
-    claim_and_append<MULTI>(P, H, Q, next_count, out, home, p, cont, finished);
+    claim_and_append<MULTI>(P, H, Q, next_count, out, home, &p, cont, finished);
   }
   block_stats(Q.stats, st, ST_BASIC);
 }
@@ -1006,7 +1029,7 @@ __global__ void __launch_bounds__(256) k_tail(scene_view S, const tpt_slot *__re
         const bool back = live && cls == Q_DIFFUSE;
         const uint32_t at = home * Q.region + wave_append(Q.counters + mc_at(parity, Q_DIFFUSE, home), back);
         if (back) {
-          store_ray(Q.mq[Q_DIFFUSE], at, p);
+          store_ray(Q.mq[Q_DIFFUSE], at, &p);
           Q.mq[Q_DIFFUSE][3][at] =
               make_float4(h.t, __uint_as_float(h.slot), __uint_as_float(h.entry), __uint_as_float(h.mat));
           live = false;
