@@ -85,6 +85,31 @@ def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
     assert np.mean(np.abs(z) > 4) < 0.005
 
 
+# The benchmark frames at their own sizes and cameras, on the same streams as
+# the oracle: random spheres 400x400 whole (16 spp), and the last sample layers
+# of the textured spheres' 1600x1600 frame (layer 299 of 300) and of the
+# Cornell 1200x1200 frame (layers 2046-2047 of 2048), so the large item, pixel
+# and sample indices of the full frames and the default launch (one pass, the
+# first fill, the 64-region queues, the fused tail) are what is compared.
+BENCH_FRAMES = [("random_spheres", 16, 0), ("random_spheres_textured", 1, 299), ("cornell555", 2, 2046)]
+
+
+@pytest.mark.parametrize("name,spp,offset", BENCH_FRAMES)
+def test_benchmark_frames_match_oracle_at_full_size(tpt, oracle, device, name, spp, offset):
+    sc = tpt.Scene.named(name)
+    w, h = sc.host.info.width, sc.host.info.height
+    s_g, _, st = sc.render(spp=spp, seed=1000, sample_offset=offset)
+    s_g = s_g.cpu().numpy()
+    s_c, _ = oracle.scene(name).render(w, h, spp, seed=1000, sample_offset=offset)
+    assert st["paths"] == w * h * spp
+    close = (np.abs(s_g - s_c) <= 1e-3 * (np.abs(s_c) + 1e-2)).all(axis=2)
+    agree = agreement_by_material(close, primary_kinds(tpt, oracle, name, w, h))
+    print(f"{name} {w}x{h}, samples {offset}..{offset + spp - 1}: pixels equal to rounding {agree}")
+    assert agree["all"] >= 0.998
+    for k, v in agree.items():
+        assert v >= 0.99, (k, v)
+
+
 @pytest.mark.parametrize("name,w,spp", [("cornell555", 40, 64), ("random_spheres", 40, 32),
                                         ("ref_random_scene", 40, 32), ("cornell_smoke", 40, 64),
                                         ("ref_two_perlin_spheres", 40, 32)])
