@@ -459,14 +459,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
   const uint32_t home = blockIdx.x % NSUB;
   uint32_t st[ST_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
-  // The sweep (and the sphere-only walk) stores the record read before the
-  // walk here and reads it back after: only the ray stays in registers across
-  // the walk.  The compiler barrier before the read-back keeps the compiler
-  // from forwarding the stored values in registers.  Measured: Cornell +4%
-  // with the sweep at 6 waves; sphere-only walk at 8 waves, 19 instead of 23
-  // spilled VGPRs, batch64 +3%.
-  constexpr bool STASH = SWEEP || SPH;
-  __shared__ float4 stash[STASH ? 3 * 256 : 1];
+  // The record read before the walk is stored here and read back after it:
+  // only the ray stays in registers across the walk.  The compiler barrier
+  // before the read-back keeps the compiler from forwarding the stored values
+  // in registers.  Measured: Cornell +4% with the sweep at 6 waves;
+  // sphere-only walk at 8 waves, 19 instead of 23 spilled VGPRs, batch64 +3%;
+  // the general walk spills 2-11 instead of 6-13 VGPRs (no change in time on
+  // box_list).
+  __shared__ float4 stash[3 * 256];
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < count;
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
       b = ld_once(&in[1][src]);
       if (SWEEP) c0 = ld_once(&in[2][src]);
     }
-    if (STASH) {
+    {
       stash[threadIdx.x] = a;
       stash[256 + threadIdx.x] = b;
       stash[512 + threadIdx.x] = c0;
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
       got = trace_world<SWEEP, SPH, MEDIA>(Sj, v3{a.x, a.y, a.z}, v3{b.x, b.y, b.z}, rc, 0.001f, kFltMax, &h,
                                            &st[ST_NODES], valid, COUNT ? &oc : nullptr);
     }
-    if (STASH) {
+    {
       asm volatile("" ::: "memory");
       a = stash[threadIdx.x];
       b = stash[256 + threadIdx.x];
