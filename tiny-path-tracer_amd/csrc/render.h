@@ -438,8 +438,9 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
 // The sweep variant parks its path record in LDS during the walk (see below)
 // and targets 6 waves per SIMD (80 VGPRs, 4 spilled): 4% faster on the Cornell
 // frame than 5 waves with the record in registers, 2% faster than 7 waves.
+constexpr uint32_t TRACE_THREADS = 256;  // k_trace's block size (its record stash holds one record per thread)
 template <bool SWEEP, bool MULTI, bool SPH = false, bool MEDIA = false, bool COUNT = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : (MEDIA ? 1 : (SWEEP ? 6 : 5))))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
+__global__ void __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per_eu(SPH ? 8 : (MEDIA ? 1 : (SWEEP ? 6 : 5))))) k_trace(scene_view S, const tpt_slot *__restrict__ slots,
                                                const tpt_entry *__restrict__ entries, render_params P,
                                                render_state Q, uint32_t parity) {
   static_assert(!(SWEEP && MULTI), "the wave-uniform sweep needs one scene");
@@ -466,7 +467,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
   // sphere-only walk at 8 waves, 19 instead of 23 spilled VGPRs, batch64 +3%;
   // the general walk spills 2-11 instead of 6-13 VGPRs (no change in time on
   // box_list).
-  __shared__ float4 stash[3 * 256];
+  __shared__ float4 stash[3 * TRACE_THREADS];
   for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
     const uint32_t qi = base + threadIdx.x;
     const bool valid = qi < count;
@@ -487,8 +488,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
     }
     {
       stash[threadIdx.x] = a;
-      stash[256 + threadIdx.x] = b;
-      stash[512 + threadIdx.x] = c0;
+      stash[TRACE_THREADS + threadIdx.x] = b;
+      stash[2 * TRACE_THREADS + threadIdx.x] = c0;
     }
     uint32_t depth = __float_as_uint(b.w) & 0xFFFFu;
     const uint32_t ent = MULTI ? __float_as_uint(b.w) >> 16 : 0u;
@@ -522,8 +523,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 
     {
       asm volatile("" ::: "memory");
       a = stash[threadIdx.x];
-      b = stash[256 + threadIdx.x];
-      c0 = stash[512 + threadIdx.x];
+      b = stash[TRACE_THREADS + threadIdx.x];
+      c0 = stash[2 * TRACE_THREADS + threadIdx.x];
       depth = __float_as_uint(b.w) & 0xFFFFu;
     }
     if (COUNT) {

@@ -757,7 +757,7 @@ void launch_tail(const bounce_launch &B, hipStream_t st, uint32_t parity, uint32
 
 template <bool COUNT>
 void launch_trace_t(const bounce_launch &B, hipStream_t st, uint32_t parity) {
-  const dim3 g(B.grid), b(256);
+  const dim3 g(B.grid), b(TRACE_THREADS);
   if (B.sweep)
     hipLaunchKernelGGL((k_trace<true, false, false, false, COUNT>), g, b, 0, st, B.S, B.S.slots, B.S.entries, B.P,
                        B.Q, parity);
