@@ -423,7 +423,8 @@ typedef struct tpt_job {
 /* Launch settings of one tpt_render_jobs call. */
 typedef struct tpt_launch {
   uint32_t queue_capacity;  /* path records in flight, at most 2^27; 0 = 2^27 */
-  uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 32 */
+  uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 64 for passes of
+                               at least 2^25 items, else 32 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to
                                even); 0 = one launch at a time */
   uint32_t flags;           /* TPT_RENDER_* flags or 0 (profiling launches one at a time) */
