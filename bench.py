@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--launch", default=None, metavar="CAP,BLOCKS,GRAPH",
                    help="batch64: use this launch (queue capacity, blocks per CU, graph iterations) instead of "
                         "autotuning")
+    p.add_argument("--queue-capacity", type=int, default=0,
+                   help="frame workloads: path records in flight (0: the library's automatic choice)")
     p.add_argument("--no-compare", action="store_true",
                    help="batch64: skip the one-call-per-render and guided comparisons")
     p.add_argument("--copy-calibration", action="store_true",
@@ -525,7 +527,7 @@ def main():
     for i in range(args.warmup):
         sum_buf.zero_()
         sq_buf.zero_()
-        sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf)
+        sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf, queue_capacity=args.queue_capacity)
     torch.cuda.synchronize()
     totals = {"segments": 0, "paths": 0, "trace_ms": 0.0, "shade_ms": 0.0, "trace_launches": 0,
               "shade_launches": 0, "light_queries": 0, "nodes_visited": 0, "nan_samples": 0, "tail_segments": 0,
@@ -534,7 +536,8 @@ def main():
     for k in range(args.steps):
         sum_buf.zero_()
         sq_buf.zero_()
-        _, _, st = sc.render(spp=spp, seed=k + 1, sum_buf=sum_buf, sumsq_buf=sq_buf, profile=True)
+        _, _, st = sc.render(spp=spp, seed=k + 1, sum_buf=sum_buf, sumsq_buf=sq_buf, profile=True,
+                             queue_capacity=args.queue_capacity)
         for key in totals:
             totals[key] += st[key]
     torch.cuda.synchronize()

@@ -326,7 +326,9 @@ typedef struct tpt_render_desc {
   uint32_t sample_offset; /* first sample index of this pass */
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
-  uint32_t queue_capacity; /* paths in flight, at most 2^27; 0 = automatic (2^27, at most the render's samples) */
+  uint32_t queue_capacity; /* paths in flight, at most 2^29; 0 = automatic (half the largest pass rounded down
+                              to a power of two, between 2^27 and 2^29, at most the pass, while the render
+                              state fits in 70% of the device memory free at the call) */
   uint32_t flags;          /* TPT_RENDER_* flags or 0 */
   uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
                               the rest with the reference sampler (0 = 2); ignored by tpt_render */
@@ -422,7 +424,7 @@ typedef struct tpt_job {
 
 /* Launch settings of one tpt_render_jobs call. */
 typedef struct tpt_launch {
-  uint32_t queue_capacity;  /* path records in flight, at most 2^27; 0 = 2^27 */
+  uint32_t queue_capacity;  /* path records in flight, at most 2^29; 0 = automatic, as for tpt_render */
   uint32_t blocks_per_cu;   /* trace/shade grid = CUs x this (256-thread blocks); 0 = 64 for passes of
                                at least 2^25 items, else 32 */
   uint32_t graph_iters;     /* bounce iterations per replay of a captured hipGraph (rounded up to

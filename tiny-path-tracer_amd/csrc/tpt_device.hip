@@ -208,10 +208,19 @@ void free_scene(tpt_scene *s) {
 // ceil(G / NSUB) blocks of a residue at most t / NSUB + t / G + ceil(G / NSUB) * B.
 // Two kernels (grid G1 of 256-thread blocks, grid G2 of k_guide_shade's
 // blocks, at most 512 threads) may append to the same ray queue, their t adding
-// up to at most `capacity`.
+// up to at most `capacity`.  G2 = 0: no guided shading (plain renders), only
+// G1 kernels append.
 size_t region_size(uint32_t capacity, uint32_t g1, uint32_t g2) {
   const size_t r1 = (g1 + NSUB - 1) / NSUB, r2 = (g2 + NSUB - 1) / NSUB;
-  return capacity / NSUB + capacity / g1 + capacity / g2 + r1 * 256 + r2 * 512 + 256;
+  return capacity / NSUB + capacity / g1 + (g2 ? capacity / g2 : 0) + r1 * 256 + r2 * 512 + 256;
+}
+
+// Device bytes of a render state (as ensure_state lays it out) for a capacity,
+// res_items samples and the launch grids.
+size_t state_bytes_for(uint32_t capacity, uint64_t res_items, uint32_t grid, uint32_t guide_grid) {
+  const size_t n = region_size(capacity, grid, guide_grid) * NSUB;
+  return 256 + COUNTER_WORDS * sizeof(uint32_t) + std::max(grid, guide_grid) * ST_COUNT * sizeof(unsigned long long) +
+         n * 16 * (2 * 3 + 3 * 4) + 40 * 256 + res_items * 3 * sizeof(float) + 256;
 }
 
 int ensure_state(tpt_scene *s, uint32_t capacity, uint64_t res_items, uint32_t grid, uint32_t guide_grid) {
@@ -823,7 +832,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   if (L.flags & ~(TPT_RENDER_PROFILE | TPT_RENDER_MEDIA_SCATTER | TPT_RENDER_NO_TAIL | TPT_RENDER_COUNT_OPS))
     return tpt::fail(TPT_ERR_INVALID, "tpt_render: unknown flags");
   if (!guide && L.guiding != 0) return tpt::fail(TPT_ERR_INVALID, "tpt_render: guiding needs tpt_render_guided");
-  if (L.queue_capacity > (1u << 27)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^27");
+  if (L.queue_capacity > (1u << 29)) return tpt::fail(TPT_ERR_INVALID, "tpt_render: queue_capacity above 2^29");
   if (L.blocks_per_cu > 64) return tpt::fail(TPT_ERR_INVALID, "tpt_render: blocks_per_cu above 64");
   if (L.graph_iters > 256) return tpt::fail(TPT_ERR_INVALID, "tpt_render: graph_iters above 256");
   uint32_t max_depth = 0;
@@ -907,11 +916,30 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   bool all_simple = true;
   for (uint32_t j = 0; j < n_jobs; ++j) all_simple = all_simple && s->simple[jobs[j].scene];
   const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * guide_shade_blocks_per_cu(all_simple);
-  // 2^27 records in flight by default (39 GB of queues): launches long enough
-  // that their ramp-up and tail cost little
-  uint32_t cap = L.queue_capacity ? L.queue_capacity : (1u << 27);
+  // queue regions and statistics rows for k_guide_shade's grid only when it runs
+  const uint32_t state_guide_grid = guide ? guide_grid : 0u;
+  // Records in flight by default: half the largest pass rounded down to a
+  // power of two, at least 2^27 and at most 2^29, while the state fits in 70%
+  // of the device memory free now (with what this scene's state already
+  // holds; never below 2^20).  More records make each bounce launch longer,
+  // so its ramp-up and tail cost less, but a fill that covers most of the pass
+  // leaves it to drain: measured on the Cornell 1200^2 frame (passes of 2^30
+  // items) 3538 / 3613 / 3667 Msamples/s at 2^27 / 2^28 / 2^29 records, on
+  // the textured spheres' 1600^2 frame (one pass of 768M items) 1736 / 1724 /
+  // 1672.
+  uint32_t cap = L.queue_capacity;
+  if (!cap) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const size_t budget = (free_b + s->state_bytes) / 10 * 7;
+    cap = 1u << 29;
+    while (cap > (1u << 27) && static_cast<uint64_t>(cap) * 2 > max_items) cap >>= 1;
+    while (cap > (1u << 20) &&
+           state_bytes_for(std::min(cap, max_items), max_items, grid, state_guide_grid) > budget)
+      cap >>= 1;
+  }
   cap = std::min(cap, max_items);
-  if ((rc = ensure_state(s, cap, max_items, grid, guide_grid)) != TPT_OK) return rc;
+  if ((rc = ensure_state(s, cap, max_items, grid, state_guide_grid)) != TPT_OK) return rc;
 
   bounce_launch B;
   std::memset(&B, 0, sizeof(B));  // the captured-graph key compares its bytes
@@ -993,7 +1021,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, ws));
-  const uint32_t n_stat = std::max(grid, guide_grid);  // rows written by block_stats
+  const uint32_t n_stat = std::max(grid, state_guide_grid);  // rows written by block_stats
   hipLaunchKernelGGL(k_zero_stats, dim3(64), dim3(256), 0, ws, s->Q.stats, n_stat * ST_COUNT);
   const int poll = static_cast<int>(kPoll);
   if (profile && s->events.size() < static_cast<size_t>(2 * poll + 1)) {
