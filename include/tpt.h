@@ -441,8 +441,8 @@ int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, cons
                     tpt_guide *guide, tpt_render_stats *stats, void *stream);
 
 /* Autotuner: times calibration renders of the jobs (into scratch sums) in
- * three stages: the queue capacities of tpt_autotune_plan at 32 blocks per CU
- * without graph replay, then grid sizes of 8 and 16 blocks per CU at the best
+ * three stages: the queue capacities of tpt_autotune_plan at 64 blocks per CU
+ * without graph replay, then grid sizes of 16 and 32 blocks per CU at the best
  * capacity, then graph replay of 8 iterations at the best of those.  Writes
  * the fastest settings into *best (its flags, guiding and guide_vertices are
  * kept).  *best_ms: that calibration's wall time. */
@@ -451,14 +451,16 @@ int tpt_autotune(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, tpt_gui
 
 /* The calibration plan of tpt_autotune (host only; no device needed).  A
  * render clips its queue capacity to its largest pass, so the candidates
- * 2^24..2^27 are clipped to the largest pass of the real render and equal
- * ones are dropped: caps[0..*n_caps) are distinct.  The calibration jobs take
+ * 2^24..2^29 are clipped to the largest pass of the real render and equal
+ * ones are dropped: caps[0..*n_caps) are distinct (at most
+ * TPT_AUTOTUNE_MAX_CAPS).  The calibration jobs take
  * min(spp, *calib_spp_out) samples each; that cap starts at calib_spp (0 = 2)
  * and doubles until a calibration pass holds at least twice the largest
  * candidate, or the calibration is the whole render.  base: the launch whose
  * guiding stride sets the real render's pass length (NULL: none). */
+#define TPT_AUTOTUNE_MAX_CAPS 6
 int tpt_autotune_plan(const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *base, uint32_t calib_spp,
-                      uint32_t caps[4], uint32_t *n_caps, uint32_t *calib_spp_out);
+                      uint32_t caps[TPT_AUTOTUNE_MAX_CAPS], uint32_t *n_caps, uint32_t *calib_spp_out);
 
 /* ------------------------------------------------------- learned guiding ---
  * No counterpart in the reference: an online-trained proposal, mixed into the

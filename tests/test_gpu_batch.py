@@ -130,7 +130,7 @@ def test_autotune_picks_a_valid_launch(tpt, pool, hosts):
     caps, _ = tpt.autotune_plan(jobs, calib_spp=1)
     print("autotune:", best.as_dict(), f"{ms:.2f} ms", "candidates", caps)
     assert best.queue_capacity in caps
-    assert best.blocks_per_cu in (8, 16, 32) and best.graph_iters in (0, 8) and ms > 0
+    assert best.blocks_per_cu in (16, 32, 64) and best.graph_iters in (0, 8) and ms > 0
     # the tuned launch gives the same sums as the default one
     jobs_a, bufs_a = batch_jobs(pool, sps)
     jobs_b, bufs_b = batch_jobs(pool, sps)

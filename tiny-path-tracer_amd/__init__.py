@@ -565,7 +565,7 @@ def autotune_plan(jobs, base=None, calib_spp=2):
     """tpt_autotune_plan (host only): the distinct queue capacities the
     autotuner times and the spp cap of its calibration jobs."""
     arr = (Job * len(jobs))(*jobs)
-    caps = (C.c_uint32 * 4)()
+    caps = (C.c_uint32 * 6)()  # TPT_AUTOTUNE_MAX_CAPS
     n, k = C.c_uint32(), C.c_uint32()
     _check(lib().tpt_autotune_plan(arr, len(jobs), C.byref(base) if base is not None else None, calib_spp, caps,
                                    C.byref(n), C.byref(k)), "tpt_autotune_plan")

@@ -1537,7 +1537,7 @@ int tpt_render_jobs(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tp
 }
 
 int tpt_autotune_plan(const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *base, uint32_t calib_spp,
-                      uint32_t caps[4], uint32_t *n_caps, uint32_t *calib_spp_out) {
+                      uint32_t caps[TPT_AUTOTUNE_MAX_CAPS], uint32_t *n_caps, uint32_t *calib_spp_out) {
   if (!jobs || n_jobs == 0 || !caps || !n_caps || !calib_spp_out)
     return tpt::fail(TPT_ERR_INVALID, "tpt_autotune_plan: bad argument");
   for (uint32_t j = 0; j < n_jobs; ++j)
@@ -1554,7 +1554,7 @@ int tpt_autotune_plan(const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *ba
   const bool recording = base && base->guiding;
   const uint64_t real_max = largest_pass(real, recording ? kPassItemsRecording : kPassItems);
   uint32_t n = 0;
-  for (uint32_t c = 1u << 24; c <= (1u << 27); c <<= 1) {
+  for (uint32_t c = 1u << 24; c <= (1u << 29); c <<= 1) {
     const uint32_t eff = static_cast<uint32_t>(std::min<uint64_t>(c, real_max));
     bool dup = false;
     for (uint32_t k = 0; k < n; ++k) dup = dup || caps[k] == eff;
@@ -1578,7 +1578,7 @@ int tpt_autotune_plan(const tpt_job *jobs, uint32_t n_jobs, const tpt_launch *ba
 int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *guide, uint32_t calib_spp,
                  tpt_launch *best, float *best_ms) {
   if (!s || !jobs || n_jobs == 0 || !best) return tpt::fail(TPT_ERR_INVALID, "tpt_autotune: bad argument");
-  uint32_t caps[4], n_caps = 0, cs_ = 0;
+  uint32_t caps[TPT_AUTOTUNE_MAX_CAPS], n_caps = 0, cs_ = 0;
   int rc = tpt_autotune_plan(jobs, n_jobs, best, calib_spp, caps, &n_caps, &cs_);
   if (rc != TPT_OK) return rc;
   // calibration jobs: at most cs_ samples each, into scratch sums
@@ -1619,9 +1619,9 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
     }
     return TPT_OK;
   };
-  for (uint32_t k = 0; k < n_caps && rc == TPT_OK; ++k) rc = trial(caps[k], 32, 0);
+  for (uint32_t k = 0; k < n_caps && rc == TPT_OK; ++k) rc = trial(caps[k], 64, 0);
   const uint32_t cap_pick = pick.queue_capacity;
-  for (uint32_t b : {8u, 16u})
+  for (uint32_t b : {16u, 32u})
     if (rc == TPT_OK) rc = trial(cap_pick, b, 0);
   if (rc == TPT_OK) rc = trial(cap_pick, pick.blocks_per_cu, 8);
   release();
