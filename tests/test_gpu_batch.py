@@ -87,14 +87,14 @@ def test_one_job_of_a_pooled_scene(tpt, pool, singles, hosts):
 
 
 def test_jobs_spanning_several_passes(tpt, pool, singles, hosts):
-    """2^30 items per pass: 1024x1024 at 1100 spp needs two passes.  Alone it
-    splits after layer 1023; in the batch the 256x256 job comes first, so its
-    layers 0-1022 share pass 1 and 1023-1099 form pass 2: a different split,
+    """2^31 items per pass: 1024x1024 at 2100 spp needs two passes.  Alone it
+    splits after layer 2047; in the batch the 256x256 job comes first, so its
+    layers 0-2046 share pass 1 and 2047-2099 form pass 2: a different split,
     the same sums."""
     import torch
 
     sps = [dict(scene=4, width=256, height=256, spp=9, seed=4, sample_offset=2, max_depth=50),
-           dict(scene=0, width=1024, height=1024, spp=1100, seed=3, sample_offset=0, max_depth=50)]
+           dict(scene=0, width=1024, height=1024, spp=2100, seed=3, sample_offset=0, max_depth=50)]
     jobs, bufs = batch_jobs(pool, sps)
     L = tpt.Launch()
     L.graph_iters = 8

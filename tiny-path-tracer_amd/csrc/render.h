@@ -4,7 +4,7 @@
 // Work: a render call covers one or more jobs (an image of one scene of the
 // scene pool: camera, resolution, samples, seed).  Its work items are the
 // jobs' (sample, pixel) pairs, job after job, sample-major within a job.  A
-// pass covers a run of whole sample layers of at most 2^30 items (2^26 while
+// pass covers a run of whole sample layers of at most 2^31 items (2^26 while
 // training rows are recorded); the pass table maps pass item w to its job,
 // pixel and sample.  A fixed number of path records is in flight; a finished
 // path writes its de_nan'ed sample to res[w] and claims the next item from a

@@ -666,10 +666,11 @@ int tpt_trace_closest(tpt_scene *s, const float *d_rays, uint64_t n, float t_min
 namespace {
 
 // Items per pass.  A pass ends by draining the queues (its last bounces run
-// at low occupancy), so passes are long: 2^30 items hold 12.9 GB of per-sample
-// results.  With training rows recorded a pass is 2^26 items (the rows of one
-// pass are kept until the next).
-constexpr uint64_t kPassItems = 1ull << 30;
+// at low occupancy) and starts with a fill, so passes are long: 2^31 items
+// hold 25.8 GB of per-sample results (item indices stay below 2^31 + 64, the
+// overshoot of a shard cursor).  With training rows recorded a pass is 2^26
+// items (the rows of one pass are kept until the next).
+constexpr uint64_t kPassItems = 1ull << 31;
 constexpr uint64_t kPassItemsRecording = 1ull << 26;
 constexpr uint32_t kMaxEnt = 0xFFFFu;        // pass entry index shares a word with the depth
 
