@@ -58,7 +58,10 @@ __host__ __device__ constexpr uint32_t mc_at(uint32_t parity, uint32_t c, uint32
   return (2 * NSUB + (parity * 3 + c) * NSUB + k) * CSTRIDE;
 }
 __host__ __device__ constexpr uint32_t cur_at(uint32_t k) { return (8 * NSUB + k) * CSTRIDE; }
-constexpr uint32_t COUNTER_WORDS = 9 * NSUB * CSTRIDE;
+// Nonzero from k_init until the pass's first k_shade: the first k_trace of a
+// pass generates the camera rays of queue 0 itself (k_init writes none).
+__host__ __device__ constexpr uint32_t first_at() { return 9 * NSUB * CSTRIDE; }
+constexpr uint32_t COUNTER_WORDS = 10 * NSUB * CSTRIDE;
 
 struct render_state {
   float4 *rq[2][3];    // ray queues by parity, 3 planes (path record); NSUB regions of `region` entries
@@ -400,11 +403,12 @@ __global__ void k_zero_stats(unsigned long long *stats, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) stats[i] = 0;
 }
 
-// First items of a pass, straight into ray queue 0 (no atomics); resets every
-// counter of the pass.  Items go in runs of 64, one wave's stores: run
-// m = w / 64 fills positions 64 (m / NSUB) .. + 63 of region m % NSUB, so
-// each store instruction writes 1 KB of consecutive addresses.
-template <bool MULTI>
+// Start of a pass: resets every counter.  Ray queue 0 holds the pass's first
+// n_first items, entry q being item q; its region counts are set as if runs
+// of 64 items had been appended region by region, and the first-trace flag
+// tells the pass's first k_trace to generate those camera rays itself instead
+// of reading them (the records of a 2^29-item fill would cost a 26 GB write
+// and read).
 __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
   const pass_hdr H = *P.hdr;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -419,12 +423,8 @@ __global__ void __launch_bounds__(256) k_init(render_params P, render_state Q) {
       v = 64 * (full / NSUB + (k < full % NSUB ? 1u : 0u)) + (part && full % NSUB == k ? part : 0u);
     }
     if (set == 8) v = shard_lo(H, k);  // shard cursors
+    if (set == 9) v = k == 0 ? 1u : 0u;  // first-trace flag
     Q.counters[w * CSTRIDE] = v;
-  }
-  for (uint32_t w = gid; w < H.n_first; w += stride) {
-    const uint32_t m = w / 64;
-    const path p = raygen_item<MULTI>(P, H, w);
-    store_ray(Q.rq[0], (m % NSUB) * Q.region + 64 * (m / NSUB) + w % 64, &p);
   }
 }
 
@@ -454,6 +454,8 @@ __global__ void __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per
   __syncthreads();
   region_prefix(Q.counters, offs, NSUB, pre);
   const uint32_t count = pre[NSUB];
+  // the pass's first launch: entry q of queue 0 is the camera ray of item q
+  const bool first = __builtin_amdgcn_readfirstlane(Q.counters[first_at()]) != 0u;
   // the next ray queue is filled by this iteration's k_shade
   if (blockIdx.x == 0 && threadIdx.x < NSUB) Q.counters[rc_at(parity ^ 1u, threadIdx.x)] = 0;
   float4 *const *in = Q.rq[parity];
@@ -479,7 +481,12 @@ __global__ void __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per
     uint32_t src = 0;
     float4 a{}, b{}, c0{};
     hit_t h{0.0f, kNone, kNone, 0};
-    if (valid) {
+    if (valid && first) {
+      const path p = raygen_item<MULTI>(P, *P.hdr, qi);
+      a = make_float4(p.o.x, p.o.y, p.o.z, __uint_as_float(p.item));
+      b = make_float4(p.d.x, p.d.y, p.d.z, __uint_as_float(p.depth | (p.ent << 16)));
+      c0 = make_float4(p.thr.x, p.thr.y, p.thr.z, __uint_as_float(p.ndiff));  // no NaN bits: radiance 0
+    } else if (valid) {
       const uint32_t r = find_region(pre, NSUB, qi);
       src = r * Q.region + (qi - pre[r]);
       a = ld_once(&in[0][src]);
@@ -533,7 +540,7 @@ __global__ void __launch_bounds__(TRACE_THREADS) __attribute__((amdgpu_waves_per
       st[ST_RECT] += oc.rect;
       st[ST_XFORM] += oc.xform;
     }
-    const float4 c = SWEEP || !valid ? c0 : ld_once(&in[2][src]);
+    const float4 c = SWEEP || !valid || first ? c0 : ld_once(&in[2][src]);
     if (valid) {
       ++st[ST_SEGMENTS];
       if (got) {
@@ -782,6 +789,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SIMPLE
   const uint32_t total = pre[NR];
   if (blockIdx.x == 0 && threadIdx.x < 3 * NSUB)
     Q.counters[mc_at(parity ^ 1u, threadIdx.x / NSUB, threadIdx.x % NSUB)] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) Q.counters[first_at()] = 0;  // the pass's first trace has run
   const uint32_t home = blockIdx.x % NSUB;
   uint32_t *next_count = Q.counters + rc_at(parity ^ 1u, home);
   float4 *const *out = Q.rq[parity ^ 1u];

@@ -1083,12 +1083,8 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
     std::memcpy(s->h_pass, pp.ents.data(), pp.ents.size() * sizeof(pass_ent));
     HIP_TRY(hipMemcpyAsync(s->d_hdr, s->h_hdr, sizeof(pass_hdr), hipMemcpyHostToDevice, ws));
     HIP_TRY(hipMemcpyAsync(s->d_pass, s->h_pass, pp.ents.size() * sizeof(pass_ent), hipMemcpyHostToDevice, ws));
-    const uint32_t n_first = s->h_hdr->n_first;
-    const dim3 ig(std::min<uint32_t>((n_first + 255) / 256 + 1, grid));
-    if (multi)
-      hipLaunchKernelGGL(k_init<true>, ig, dim3(256), 0, ws, P, s->Q);
-    else
-      hipLaunchKernelGGL(k_init<false>, ig, dim3(256), 0, ws, P, s->Q);
+    // counters only: the first k_trace generates the first items' rays
+    hipLaunchKernelGGL(k_init, dim3((COUNTER_WORDS / CSTRIDE + 255) / 256), dim3(256), 0, ws, P, s->Q);
     // A path has at most max_depth + 1 segments and each bounce retires at
     // least one path per live record, so the loop is bounded.
     const uint64_t max_iter = (static_cast<uint64_t>(pp.n_items) / cap + 2) * (max_depth + 2ull) + 64;
@@ -1453,8 +1449,7 @@ int render_noalloc(tpt_scene *s, const tpt_camera *camera, const tpt_render_desc
     e.n_layers = take;
     e.acc_begin = 0;
     hipLaunchKernelGGL(k_noalloc_pass, dim3(1), dim3(64), 0, ws, d_jobs, jd, d_hdr, h, d_pass, e);
-    hipLaunchKernelGGL(k_init<false>, dim3(std::min<uint32_t>((h.n_first + 255) / 256 + 1, grid)), dim3(256), 0,
-                       ws, P, Q);
+    hipLaunchKernelGGL(k_init, dim3((COUNTER_WORDS / CSTRIDE + 255) / 256), dim3(256), 0, ws, P, Q);
     const uint64_t max_iter = (static_cast<uint64_t>(h.n_items) / cap + 2) * (desc->max_depth + 2ull) + 64;
     drained = false;
     bool exhausted = h.n_items <= cap;
