@@ -90,7 +90,8 @@ def test_same_streams_match_oracle(tpt, oracle, device, name, w, spp):
 # of the textured spheres' 1600x1600 frame (layer 299 of 300) and of the
 # Cornell 1200x1200 frame (layers 2046-2047 of 2048), so the large item, pixel
 # and sample indices of the full frames and the default launch (one pass, the
-# first fill, the 64-region queues, the fused tail) are what is compared.
+# first trace generating the camera rays, the 64-region queues, the fused
+# tail) are what is compared.
 BENCH_FRAMES = [("random_spheres", 16, 0), ("random_spheres_textured", 1, 299), ("cornell555", 2, 2046)]
 
 
