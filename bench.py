@@ -466,14 +466,16 @@ def guided(tpt, torch, args):
     sum_buf = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
     sq_buf = torch.zeros_like(sum_buf)
     for i in range(args.warmup):
-        sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g)
+        sc.render(spp=spp, seed=1000 + i, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g,
+                  queue_capacity=args.queue_capacity)
     torch.cuda.synchronize()
     segs = paths = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         sum_buf.zero_()
         sq_buf.zero_()
-        _, _, st = sc.render(spp=spp, seed=k + 1, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g)
+        _, _, st = sc.render(spp=spp, seed=k + 1, sum_buf=sum_buf, sumsq_buf=sq_buf, guide=g,
+                             queue_capacity=args.queue_capacity)
         segs += st["segments"]
         paths += st["paths"]
     torch.cuda.synchronize()

@@ -326,9 +326,11 @@ typedef struct tpt_render_desc {
   uint32_t sample_offset; /* first sample index of this pass */
   float background;       /* 0: miss returns black (reference); > 0: sky gradient times this */
   uint32_t guiding;       /* tpt_render: must be 0; tpt_render_guided: training-record stride (0 = none) */
-  uint32_t queue_capacity; /* paths in flight, at most 2^29; 0 = automatic (half the largest pass rounded down
-                              to a power of two, between 2^27 and 2^29, at most the pass, while the render
-                              state fits in 70% of the device memory free at the call) */
+  uint32_t queue_capacity; /* paths in flight, at most 2^29; 0 = automatic (2^29 for a single scene traced by
+                              the wave-uniform sweep: at most 64 slots, no motion, media or wrapped trees;
+                              otherwise half the largest pass rounded down to a power of two between 2^27 and
+                              2^29; at most the pass, while the render state fits in 70% of the device memory
+                              free at the call) */
   uint32_t flags;          /* TPT_RENDER_* flags or 0 */
   uint32_t guide_vertices; /* tpt_render_guided: diffuse vertices per path sampled with the guide,
                               the rest with the reference sampler (0 = 2); ignored by tpt_render */

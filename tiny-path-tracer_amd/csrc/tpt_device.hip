@@ -919,22 +919,25 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   const uint32_t guide_grid = static_cast<uint32_t>(s->n_cu) * guide_shade_blocks_per_cu(all_simple);
   // queue regions and statistics rows for k_guide_shade's grid only when it runs
   const uint32_t state_guide_grid = guide ? guide_grid : 0u;
-  // Records in flight by default: half the largest pass rounded down to a
-  // power of two, at least 2^27 and at most 2^29, while the state fits in 70%
-  // of the device memory free now (with what this scene's state already
-  // holds; never below 2^20).  More records make each bounce launch longer,
-  // so its ramp-up and tail cost less, but a fill that covers most of the pass
-  // leaves it to drain: measured on the Cornell 1200^2 frame (passes of 2^30
-  // items) 3538 / 3613 / 3667 Msamples/s at 2^27 / 2^28 / 2^29 records, on
-  // the textured spheres' 1600^2 frame (one pass of 768M items) 1736 / 1724 /
-  // 1672.
+  // Records in flight by default: 2^29 for single-scene renders that sweep
+  // their tree (small scenes), half the largest pass rounded down to a power
+  // of two (at least 2^27, at most 2^29) for the others, at most the pass, and
+  // while the state fits in 70% of the device memory free now (with what this
+  // scene's state already holds; never below 2^20).  More records make each
+  // bounce launch longer, so its ramp-up and tail cost less; for per-lane
+  // walks a fill that covers most of the pass costs more than that.  Measured
+  // (Msamples/s at 2^27 / 2^28 / 2^29 records): the Cornell 1200^2 frame
+  // (passes of 2^30 items) 3538 / 3613 / 3667, its guided frame (one pass of
+  // 368M items) 2669 / 2689 / 2718, the textured spheres' 1600^2 frame (one
+  // pass of 768M items, per-lane walks) 1736 / 1724 / 1672.
+  const bool sweep_scene = !multi && s->sweep[jobs[0].scene];
   uint32_t cap = L.queue_capacity;
   if (!cap) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     const size_t budget = (free_b + s->state_bytes) / 10 * 7;
     cap = 1u << 29;
-    while (cap > (1u << 27) && static_cast<uint64_t>(cap) * 2 > max_items) cap >>= 1;
+    while (!sweep_scene && cap > (1u << 27) && static_cast<uint64_t>(cap) * 2 > max_items) cap >>= 1;
     while (cap > (1u << 20) &&
            state_bytes_for(std::min(cap, max_items), max_items, grid, state_guide_grid) > budget)
       cap >>= 1;
@@ -948,7 +951,7 @@ int render_core(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, const tpt_la
   B.Q = s->Q;
   B.grid = grid;
   B.guide_grid = guide_grid;
-  B.sweep = !multi && s->sweep[jobs[0].scene];
+  B.sweep = sweep_scene;
   B.sph = true;
   B.simple = true;
   for (uint32_t j = 0; j < n_jobs; ++j) {
