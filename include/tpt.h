@@ -443,7 +443,8 @@ int tpt_render_jobs(tpt_scene *scene, const tpt_job *jobs, uint32_t n_jobs, cons
                     tpt_guide *guide, tpt_render_stats *stats, void *stream);
 
 /* Autotuner: times calibration renders of the jobs (into scratch sums) in
- * three stages: the queue capacities of tpt_autotune_plan at 64 blocks per CU
+ * three stages: the queue capacities of tpt_autotune_plan (those whose render
+ * state fits in 70% of the device memory free at the call) at 64 blocks per CU
  * without graph replay, then grid sizes of 16 and 32 blocks per CU at the best
  * capacity, then graph replay of 8 iterations at the best of those.  Writes
  * the fastest settings into *best (its flags, guiding and guide_vertices are

@@ -1580,6 +1580,24 @@ int tpt_autotune(tpt_scene *s, const tpt_job *jobs, uint32_t n_jobs, tpt_guide *
   uint32_t caps[TPT_AUTOTUNE_MAX_CAPS], n_caps = 0, cs_ = 0;
   int rc = tpt_autotune_plan(jobs, n_jobs, best, calib_spp, caps, &n_caps, &cs_);
   if (rc != TPT_OK) return rc;
+  {
+    // Candidates are given to the trials explicitly, past the render's own
+    // memory check: a capacity whose render state for the real render would
+    // not fit in 70% of the device memory free now (with what this scene's
+    // state already holds) is not timed.  The smallest is always kept.
+    uint64_t real_max = 0;
+    for (const pass_plan &pp : plan_passes(jobs, n_jobs, best->guiding ? kPassItemsRecording : kPassItems))
+      real_max = std::max<uint64_t>(real_max, pp.n_items);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const size_t budget = (free_b + s->state_bytes) / 10 * 7;
+    const uint32_t g1 = static_cast<uint32_t>(s->n_cu) * 64u;                  // the largest grid tried
+    const uint32_t g2 = guide ? static_cast<uint32_t>(s->n_cu) * 3u : 0u;  // k_guide_shade's largest grid
+    uint32_t kept = 0;
+    for (uint32_t k = 0; k < n_caps; ++k)
+      if (kept == 0 || state_bytes_for(caps[k], real_max, g1, g2) <= budget) caps[kept++] = caps[k];
+    n_caps = kept;
+  }
   // calibration jobs: at most cs_ samples each, into scratch sums
   std::vector<tpt_job> cj(jobs, jobs + n_jobs);
   std::vector<float *> scratch(n_jobs, nullptr);
