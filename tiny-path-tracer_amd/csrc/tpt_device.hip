@@ -1313,7 +1313,8 @@ noalloc_layout noalloc_plan(uint32_t grid) {
   // blocks append (k_trace to the material queues, k_shade to the next ray
   // queue), at most one entry per live record, so a region holds at most
   // t/NSUB + t/G + ceil(G/NSUB)*256 entries (region_size without the guided
-  // shading term); k_init's runs of 64 fit in the +256.
+  // shading term); the first items' region counts that k_init sets (runs of
+  // 64 entries) fit in the +256.
   const uint32_t cap = TPT_NOALLOC_PATHS;
   L.region = cap / NSUB + cap / grid + (grid + NSUB - 1) / NSUB * 256 + 256;
   const size_t n = static_cast<size_t>(L.region) * NSUB;
